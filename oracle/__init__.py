@@ -19,7 +19,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle_psim.so")
 _lib = None
 
 COUNTER_NAMES = ["phonon_steps", "scatters", "impacts", "relocations", "absorbed", "expired",
-                 "not_started", "tallies"]
+                 "not_started", "tallies", "stalled"]
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I64 = ctypes.POINTER(ctypes.c_int64)

@@ -46,6 +46,9 @@
 #define INV_EPS 0.0001
 #define INV_MAX_ITERS 40
 #define MAX_SUBS 16
+/* Loop iterations allowed inside one measurement step before the phonon is removed; the
+ * reference has no bound, the GPU engine applies the same one so every wave terminates. */
+#define MAX_EVENTS_PER_STEP 1000000
 
 enum { SIM_SS = 0, SIM_PERIODIC = 1, SIM_TRANSIENT = 2 };
 enum { POL_LA = 0, POL_TA = 1 };
@@ -1260,7 +1263,13 @@ static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint3
         double relax[3] = { 0, 0, 0 };
         double time_to_scatter = 0.;
         double time_to_measurement = 0.;
+        int events = 0;
         for (;;) {
+            if (++events > MAX_EVENTS_PER_STEP) {
+                ta->counters[OR_CNT_STALLED]++;
+                cause = 4;
+                break;
+            }
             if (time_to_scatter <= 0.) {
                 transport_rates(m, p, relax);
                 time_to_scatter = SCALING_FACTOR * -psim_log(psim_urand(&st)) / (0. + relax[0] + relax[1] + relax[2]);
@@ -1281,6 +1290,7 @@ static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint3
                 time_to_scatter -= drift_time;
                 if (time_to_measurement == 0.) {
                     ta->counters[OR_CNT_PHONON_STEPS]++;
+                    events = 0;
                     if (++step < N) {
                         p->lifestep = step;
                         psim_stream_set_step(&st, (uint32_t)step);

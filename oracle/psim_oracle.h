@@ -63,6 +63,7 @@ enum {
     OR_CNT_EXPIRED,
     OR_CNT_NOT_STARTED,
     OR_CNT_TALLIES,
+    OR_CNT_STALLED,
     OR_NUM_COUNTERS
 };
 
@@ -70,7 +71,7 @@ enum {
  * called with the same seed).  counts[S*R] (int64) and flux[S*R*2] receive
  * the tallies, sensor-major in sensor insertion order; counters[OR_NUM_COUNTERS].
  * fate (optional, may be NULL): per phonon id, int64 step at death, int32
- * cause (1 absorbed, 2 expired, 3 not started) and final x, y. */
+ * cause (1 absorbed, 2 expired, 3 not started, 4 stalled) and final x, y. */
 int or_run(or_model* m, uint64_t seed, int nthreads, int64_t* counts, double* flux, int64_t* counters,
     int64_t* fate_step, int32_t* fate_cause, double* fate_xy);
 
