@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X phonon engine on psim's scaled steady-state workload.
+
+Workload (one "step" = one complete simulation run: emission of the whole budget, then every
+phonon carried through 5000 measurement steps of 2 ps, tallies reduced on the GPU):
+the conforming L-shaped silicon structure (40 sensors, 80 triangles, walls 320 / 280 K,
+specularity 0.3, T_init 300 K), full (non-deviational) mode, 1e7 phonons, FP64 tallies.
+
+  value      phonon-steps per second of the timed runs (a phonon-step is one live phonon carried
+             across one measurement interval: the reference's ++step at
+             psim/src/modelSimulator.cpp:182-183), with the model already resident in HBM.
+  roofline   the transport kernel against HBM: 120 bytes per phonon-step (the phonon record
+             read and written once per step in a step-synchronous design) over its HIP-event
+             time on the engine stream.
+  cpu_baseline  the reference psim (oracle/_ref/psim_ref, built from the reference sources) on
+             a bounded sample of the same workload (1e5 phonons), one single-threaded process on
+             this host; its phonon-steps are counted by the CPU oracle on the same model.
+
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BYTES_PER_PHONON_STEP = 120.0
+HBM_PEAK_GBPS = 8000.0
+
+
+def cpu_baseline(sample_phonons: int) -> dict | None:
+    """Time the reference build (or, without it, the oracle) on a sample of the workload."""
+    import oracle
+    from oracle import refmodel
+    from psim_amd import configs
+
+    cfg = configs.lshape_full(num_phonons=sample_phonons)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    counted = oracle.Model(cfg).run(0, nthreads=threads).counters["phonon_steps"]
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "psim_ref")
+    sample = (f"L-shape full mode, {sample_phonons} phonons x 5000 steps of 2 ps, one process; "
+              f"{counted} phonon-steps counted by the oracle on the same model")
+    if os.path.exists(ref_bin):
+        with tempfile.TemporaryDirectory() as wd:
+            txt = os.path.join(wd, "model.txt")
+            out = os.path.join(wd, "out.json")
+            with open(txt, "w") as f:
+                f.write(refmodel.json_to_text(cfg))
+            subprocess.run([ref_bin, txt, out], check=True, stdout=subprocess.DEVNULL, timeout=600)
+            with open(out) as f:
+                wall = json.load(f)["wall_s"]
+        return {"value": counted / wall, "unit": "phonon-steps/s", "cores": 1, "kind": "reference",
+                "sample": sample, "wall_s": wall}
+    t0 = time.perf_counter()
+    oracle.Model(cfg).run(0, nthreads=1)
+    wall = time.perf_counter() - t0
+    return {"value": counted / wall, "unit": "phonon-steps/s", "cores": 1, "kind": "port", "sample": sample,
+            "wall_s": wall}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--phonons", type=int, default=10_000_000)
+    ap.add_argument("--cpu-sample-phonons", type=int, default=100_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X (the multi-GPU design, sharding "
+              "phonon ids and all-reducing the tallies, is described in DESIGN.md and not run here).",
+              file=sys.stderr)
+        return 2
+
+    import psim_amd
+    from psim_amd import configs
+
+    cfg = configs.lshape_full(num_phonons=args.phonons)
+    engine = psim_amd.Engine(0)
+    model = psim_amd.Model(cfg)
+    _, total = model.upload(engine, 0)
+    for w in range(args.warmup):
+        engine.run_resident(10_000 + w)
+    steps_total = 0
+    kernel_ms = 0.0
+    launches = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        engine.run_resident(k)
+        _, _, counters = engine.fetch(model.num_sensors, model.num_record)
+        tm = engine.last_timing()
+        steps_total += counters["phonon_steps"]
+        kernel_ms += tm["step_kernel_ms"]
+        launches += tm["step_launches"]
+    wall = time.perf_counter() - t0
+    value = steps_total / wall
+    achieved = BYTES_PER_PHONON_STEP * steps_total / (kernel_ms / 1e3) / 1e9
+    line = {
+        "metric": "phonon-steps/s",
+        "value": value,
+        "unit": "phonon-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "L-shaped Si (40 sensors, 80 triangles), full-mode steady state, walls 320/280 K, "
+                        "specularity 0.3",
+            "phonons": total,
+            "measurement_steps": 5000,
+            "dt_ps": 2.0,
+            "tallies": "fp64",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "transport_kernel",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "bytes_per_phonon_step": BYTES_PER_PHONON_STEP,
+            "kernel_ms_per_launch": kernel_ms / max(1, launches),
+            "launches": launches,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_sample_phonons)
+    print(json.dumps(line), flush=True)
+    engine.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -236,6 +236,10 @@ int psim_model_results(const psim_model* model, int64_t run, double* out, double
 int psim_model_prepare(psim_model* model, uint64_t seed, double* eff_energy, int64_t* total_phonons, int64_t* counts,
     int32_t counts_cap, int32_t* n_sources);
 
+/* psim_model_prepare followed by psim_gpu_upload of the flattened model to `ctx`, so that
+ * psim_gpu_run_resident can then be timed with every input already in HBM. */
+int psim_model_upload(psim_model* model, psim_gpu_ctx* ctx, uint64_t seed, double* eff_energy, int64_t* total_phonons);
+
 /* Budget of the last prepared run: energy per phonon and phonon count. */
 int psim_model_budget(const psim_model* model, double* eff_energy, int64_t* total_phonons);
 

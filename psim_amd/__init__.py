@@ -76,6 +76,7 @@ def lib():
         "psim_model_budget": (ctypes.c_int, [_VP, _D, _I64]),
         "psim_model_prepare": (ctypes.c_int, [_VP, ctypes.c_uint64, _D, _I64, _I64, ctypes.c_int32, _I32]),
         "psim_model_export": (ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_double]),
+        "psim_model_upload": (ctypes.c_int, [_VP, _VP, ctypes.c_uint64, _D, _I64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -203,6 +204,14 @@ class Model:
         counts = np.zeros(n.value, np.int64)
         self._check(self._L.psim_model_prepare(self._h, int(seed), None, None, _p(counts, _I64), n.value, None))
         return eff.value, total.value, counts
+
+    def upload(self, engine: Engine, seed: int):
+        """Prepare one run and upload it to the engine: (energy per phonon, total phonons)."""
+        eff = ctypes.c_double()
+        total = ctypes.c_int64()
+        self._check(self._L.psim_model_upload(self._h, engine.handle, int(seed), ctypes.byref(eff),
+                                              ctypes.byref(total)))
+        return eff.value, total.value
 
     def run(self, engine: Engine, seed: int) -> None:
         self._check(self._L.psim_model_run(self._h, engine.handle, int(seed)))

@@ -108,6 +108,16 @@ int psim_model_prepare(psim_model* m, uint64_t seed, double* eff_energy, int64_t
     } catch (const std::exception& e) { return model_fail(m, PSIM_E_MODEL, e.what()); }
 }
 
+int psim_model_upload(psim_model* m, psim_gpu_ctx* ctx, uint64_t seed, double* eff_energy, int64_t* total_phonons) {
+    if (!m || !ctx) { return PSIM_E_INVALID; }
+    try {
+        const double eff = m->model->prepareAndUpload(ctx, seed);
+        if (eff_energy) { *eff_energy = eff; }
+        if (total_phonons) { *total_phonons = m->model->simulator().totalPhonons(); }
+        return PSIM_OK;
+    } catch (const std::exception& e) { return model_fail(m, PSIM_E_DEVICE, e.what()); }
+}
+
 int psim_model_export(const psim_model* m, const char* json_path, double time_s) {
     if (!m || !json_path) { return PSIM_E_INVALID; }
     try {

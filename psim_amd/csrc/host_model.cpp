@@ -811,6 +811,12 @@ double Model::prepareBudget(uint64_t seed) {
     return prepare(seed);
 }
 
+double Model::prepareAndUpload(psim_gpu_ctx* ctx, uint64_t seed) {
+    if (!ctx) { throw std::runtime_error("no engine context"); }
+    simulator_.setEngine(ctx);
+    return prepare(seed);
+}
+
 void Model::runSimulation(psim_gpu_ctx* ctx, uint64_t seed) {  // model.cpp:141-182
     if (!ctx) { throw std::runtime_error("no engine context"); }
     runs_.clear();

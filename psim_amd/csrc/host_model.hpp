@@ -241,6 +241,9 @@ public:
     // Pre-run state of one run (bounds, tables, energy per phonon, emission budget) without an
     // engine; returns the budget's energy per phonon.
     double prepareBudget(uint64_t seed);
+    // The same pre-run state uploaded to the engine of `ctx` (for timing runs with the model
+    // resident in HBM); returns the budget's energy per phonon.
+    double prepareAndUpload(psim_gpu_ctx* ctx, uint64_t seed);
     const GpuModelSimulator& simulator() const noexcept { return simulator_; }
 
     std::size_t numRuns() const noexcept { return num_runs_; }
