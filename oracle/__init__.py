@@ -65,6 +65,16 @@ def _load():
     lib.or_num_temps.argtypes = [vp, ctypes.c_int]
     lib.or_temps.argtypes = [vp, ctypes.c_int, _D, _D]
     lib.or_cell_geometry.argtypes = [vp, ctypes.c_int, _D, _D, _D, _D, _I32, _I32]
+    _U32 = ctypes.POINTER(ctypes.c_uint32)
+    lib.or_math.argtypes = [ctypes.c_int, _D, _D, ctypes.c_int64]
+    lib.or_philox.argtypes = [_U32, _U32, _U32]
+    lib.or_philox.restype = None
+    lib.or_stream.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int64, _D]
+    lib.or_stream.restype = None
+    lib.or_freq_index.argtypes = [_D, _D, ctypes.c_double, ctypes.c_double, _I64, _I32]
+    lib.or_freq_index.restype = None
+    lib.or_directions.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _D]
+    lib.or_directions.restype = None
     _lib = lib
     return lib
 
@@ -94,8 +104,10 @@ class Model:
         lib = _load()
         self._lib = lib
         s = model["settings"]
+        # InputManager passes step_interval only for periodic and transient runs (inputManager.cpp:77-78)
+        step_interval = s.get("step_interval", 0) if int(s["sim_type"]) in (1, 2) else 0
         settings = np.array([s.get("num_runs", 1), s["num_measurements"], s["num_phonons"], s["sim_time"],
-                             s["t_eq"], s["sim_type"], s.get("step_interval", 0),
+                             s["t_eq"], s["sim_type"], step_interval,
                              1.0 if s.get("phasor_sim", False) else 0.0], dtype=np.float64)
         err = ctypes.create_string_buffer(512)
         h = lib.or_model_new(_p(settings, _D), err, 512)
@@ -220,3 +232,51 @@ class Model:
         self._lib.or_cell_geometry(self._h, int(cell), _p(area, _D), _p(lines, _D), _p(normals, _D), _p(spec, _D),
                                    _p(nt, _I32), _p(ne, _I32))
         return {"area": float(area[0]), "lines": lines, "normals": normals, "spec": spec, "n_trans": nt, "n_emit": ne}
+
+
+# ---- test hooks -------------------------------------------------------------------------
+
+MATH_FUNCTIONS = {"log": 0, "exp": 1, "sinh": 2, "cos2pi": 3, "pow4": 4}
+
+
+def math_fn(name: str, x: np.ndarray) -> np.ndarray:
+    lib = _load()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    if lib.or_math(MATH_FUNCTIONS[name], _p(x, _D), _p(y, _D), x.size) != 0:
+        raise OracleError("bad function")
+    return y
+
+
+def philox(ctr, key) -> np.ndarray:
+    lib = _load()
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, np.uint32)
+    U32 = ctypes.POINTER(ctypes.c_uint32)
+    lib.or_philox(c.ctypes.data_as(U32), k.ctypes.data_as(U32), out.ctypes.data_as(U32))
+    return out
+
+
+def stream(seed: int, tag: int, ident: int, step: int, n: int) -> np.ndarray:
+    lib = _load()
+    out = np.zeros(n, np.float64)
+    lib.or_stream(int(seed), int(tag), int(ident), int(step), int(n), _p(out, _D))
+    return out
+
+
+def freq_index(cdf, frac, u1: float, u2: float):
+    lib = _load()
+    c = np.ascontiguousarray(cdf, dtype=np.float64)
+    f = np.ascontiguousarray(frac, dtype=np.float64)
+    b = np.zeros(1, np.int64)
+    p = np.zeros(1, np.int32)
+    lib.or_freq_index(_p(c, _D), _p(f, _D), float(u1), float(u2), _p(b, _I64), _p(p, _I32))
+    return int(b[0]), int(p[0])
+
+
+def directions(kind: int, seed: int, n: int, normal=(1.0, 0.0), dirs=None) -> np.ndarray:
+    lib = _load()
+    out = np.zeros((n, 2), np.float64) if dirs is None else np.ascontiguousarray(dirs, dtype=np.float64).copy()
+    lib.or_directions(int(kind), int(seed), int(n), float(normal[0]), float(normal[1]), _p(out, _D))
+    return out

@@ -1031,11 +1031,11 @@ typedef struct {
     int64_t counters[OR_NUM_COUNTERS];
 } tally_t;
 
-/* Material::freqIndex (bisection returning `high`), material.cpp:64-75 */
-static void freq_index(const table_t* t, psim_stream* st, int64_t* index, int* polar) {
+/* Material::freqIndex (bisection returning `high`), material.cpp:64-75, given its two draws:
+ * `rand` for the bisection and u2 for the polarization. */
+static void freq_index_draws(const table_t* t, double rand, double u2, int64_t* index, int* polar) {
     int64_t low = 0, high = NBINS - 1;
     int64_t mid = low + (high - low) / 2;
-    const double rand = psim_urand(st);
     while (high - low > 1) {
         if (rand < t->cdf[mid]) {
             high = mid;
@@ -1045,7 +1045,13 @@ static void freq_index(const table_t* t, psim_stream* st, int64_t* index, int* p
         mid = (low + high) / 2;
     }
     *index = high;
-    *polar = (psim_urand(st) <= t->frac[high]) ? POL_LA : POL_TA;
+    *polar = (u2 <= t->frac[high]) ? POL_LA : POL_TA;
+}
+
+static void freq_index(const table_t* t, psim_stream* st, int64_t* index, int* polar) {
+    const double rand = psim_urand(st);
+    const double u2 = psim_urand(st);
+    freq_index_draws(t, rand, u2, index, polar);
 }
 
 /* SensorController::initialUpdate / scatterUpdate with Material::getFreq / getVel and
@@ -1644,4 +1650,69 @@ int or_cell_geometry(const or_model* m, int ci, double* area, double* lines, dou
         n_emit[b] = c->b[b].n_emit;
     }
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Hooks for the oracle's own pinning tests */
+
+int or_math(int fn, const double* x, double* y, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        switch (fn) {
+        case 0: y[i] = psim_log(x[i]); break;
+        case 1: y[i] = psim_exp(x[i]); break;
+        case 2: y[i] = psim_sinh(x[i]); break;
+        case 3: y[i] = psim_cos2pi(x[i]); break;
+        case 4: y[i] = psim_pow4(x[i]); break;
+        default: return -1;
+        }
+    }
+    return 0;
+}
+
+void or_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    psim_u32x4 c;
+    for (int i = 0; i < 4; ++i) { c.v[i] = ctr[i]; }
+    const psim_u32x4 r = psim_philox4x32_10(c, key[0], key[1]);
+    for (int i = 0; i < 4; ++i) { out[i] = r.v[i]; }
+}
+
+void or_stream(uint64_t seed, uint32_t tag, uint32_t id, uint32_t step, int64_t n, double* out) {
+    psim_stream st;
+    psim_stream_init(&st, seed, tag, id, step);
+    for (int64_t i = 0; i < n; ++i) { out[i] = psim_urand(&st); }
+}
+
+/* Material::freqIndex with the draws u1 (bisection) and u2 (polarization). */
+void or_freq_index(const double* cdf, const double* frac, double u1, double u2, int64_t* bin, int32_t* pol) {
+    table_t t;
+    memcpy(t.cdf, cdf, sizeof t.cdf);
+    memcpy(t.frac, frac, sizeof t.frac);
+    int64_t index = 0;
+    int polar = 0;
+    freq_index_draws(&t, u1, u2, &index, &polar);
+    *bin = index;
+    *pol = polar;
+}
+
+/* Directions: kind 0 Phonon::setRandDirection, kind 1 Surface::redirectPhonon about (nx, ny),
+ * kind 2 the specular branch of Surface::boundaryHandlePhonon applied to (dx, dy) = out[2i..]. */
+void or_directions(int kind, uint64_t seed, int64_t n, double nx, double ny, double* out) {
+    const pt nrm = { nx, ny };
+    for (int64_t i = 0; i < n; ++i) {
+        phonon_t p;
+        memset(&p, 0, sizeof p);
+        psim_stream st;
+        psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, (uint32_t)i, 7);
+        if (kind == 0) {
+            set_rand_direction(&p, &st);
+        } else if (kind == 1) {
+            redirect(nrm, &p, &st);
+        } else {
+            p.dx = out[2 * i];
+            p.dy = out[2 * i + 1];
+            boundary_handle(nrm, 1.0, &p, &st);
+        }
+        out[2 * i] = p.dx;
+        out[2 * i + 1] = p.dy;
+    }
 }

@@ -96,6 +96,15 @@ int or_temps(const or_model* m, int material, double* temps, double* base_sums);
 int or_cell_geometry(const or_model* m, int cell, double* area, double* lines, double* normals, double* spec,
     int32_t* n_trans, int32_t* n_emit);
 
+/* Test hooks: the shared transcendentals (fn 0 log, 1 exp, 2 sinh, 3 cos2pi, 4 pow4), one
+ * Philox4x32-10 block, a uniform stream, the frequency-bin bisection with given draws, and
+ * direction sampling (0 isotropic, 1 Lambertian about n, 2 specular reflection of out[]). */
+int or_math(int fn, const double* x, double* y, int64_t n);
+void or_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
+void or_stream(uint64_t seed, uint32_t tag, uint32_t id, uint32_t step, int64_t n, double* out);
+void or_freq_index(const double* cdf, const double* frac, double u1, double u2, int64_t* bin, int32_t* pol);
+void or_directions(int kind, uint64_t seed, int64_t n, double nx, double ny, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -102,6 +102,24 @@ bool lineOverlaps(const Line& s, const Line& o) {  // :56-71
     return false;
 }
 
+std::string fmtPoint(Point p) {  // geometry.cpp:29-32
+    std::ostringstream os;
+    os << "Point (" << p.x << ", " << p.y << ')';
+    return os.str();
+}
+
+std::string fmtTriangle(Point a, Point b, Point c) {  // geometry.cpp:268-271
+    return "Triangle: [" + fmtPoint(a) + ", " + fmtPoint(b) + ", " + fmtPoint(c) + "]";
+}
+
+// CompositeSurfaceError message, compositeSurface.cpp:100-108
+[[noreturn]] void surfaceConflict(const Line& main, const Line& inc) {
+    throw std::runtime_error(
+        "An existing surface conflicts with the location of the incoming surface.\nExisting surface points are: "
+        + fmtPoint(main.p1) + ' ' + fmtPoint(main.p2) + "\nIncoming surface points are: " + fmtPoint(inc.p1) + ' '
+        + fmtPoint(inc.p2) + '\n');
+}
+
 bool triValid(Point p1, Point p2, Point p3) {  // :150-158
     return !((std::fabs(slopeOf(p1, p2)) == std::fabs(slopeOf(p2, p3))
                  && std::fabs(slopeOf(p2, p3)) == std::fabs(slopeOf(p3, p1)))
@@ -194,20 +212,14 @@ Line::Line(Point a, Point b) : p1(a), p2(b) {  // geometry.cpp:45-53
     bl = { blx, bly };
     tr = { trx, try_ };
     length = std::sqrt((b.x - a.x) * (b.x - a.x) + (b.y - a.y) * (b.y - a.y));
-    if (ptEq(a, b)) {
-        std::ostringstream os;
-        os << "Cannot create a line using 2 identical points -> Point (" << a.x << ", " << a.y << ") Point (" << b.x
-           << ", " << b.y << ")";
-        throw std::runtime_error(os.str());
+    if (ptEq(a, b)) {  // LineError, geometry.cpp:333-338
+        throw std::runtime_error("Cannot create a line using 2 identical points -> " + fmtPoint(a) + ' ' + fmtPoint(b));
     }
 }
 
 Triangle::Triangle(Point a, Point b, Point c) : p1(a), p2(b), p3(c) {
-    if (!triValid(a, b, c)) {
-        std::ostringstream os;
-        os << "These 3 points do not allow for a valid triangle -> Triangle: [Point (" << a.x << ", " << a.y
-           << "), Point (" << b.x << ", " << b.y << "), Point (" << c.x << ", " << c.y << ")]";
-        throw std::runtime_error(os.str());
+    if (!triValid(a, b, c)) {  // TriangleError, geometry.cpp:340-345
+        throw std::runtime_error("These 3 points do not allow for a valid triangle -> " + fmtTriangle(a, b, c));
     }
 }
 
@@ -334,19 +346,11 @@ psim_material Material::descriptor() const {
 
 bool CompositeSurface::verify(const Line& l) const {  // compositeSurface.cpp:68-87
     if (!lineContains(line, l)) { return false; }
-    auto conflict = [&](const Line& existing) {
-        std::ostringstream os;
-        os << "An existing surface conflicts with the location of the incoming surface.\n"
-           << "Existing surface points are: Point (" << existing.p1.x << ", " << existing.p1.y << ") Point ("
-           << existing.p2.x << ", " << existing.p2.y << ")\nIncoming surface points are: Point (" << l.p1.x << ", "
-           << l.p1.y << ") Point (" << l.p2.x << ", " << l.p2.y << ")\n";
-        throw std::runtime_error(os.str());
-    };
     for (const auto& t : transitions) {
-        if (lineOverlaps(l, t.line)) { conflict(t.line); }
+        if (lineOverlaps(l, t.line)) { surfaceConflict(t.line, l); }
     }
     for (const auto& e : emits) {
-        if (lineOverlaps(l, e.line)) { conflict(e.line); }
+        if (lineOverlaps(l, e.line)) { surfaceConflict(e.line, l); }
     }
     return true;
 }
@@ -646,7 +650,7 @@ void Model::addSensor(std::size_t id, const std::string& material_name, double t
     for (const auto& m : materials_) {
         if (m.first == material_name) { mat = m.second.get(); }
     }
-    if (!mat) { throw std::runtime_error("map::at"); }  // materials_.at(material_name)
+    if (!mat) { throw std::runtime_error("_Map_base::at"); }  // std::unordered_map::at, model.cpp:91
     if (sensors_.size() == sensors_.capacity()) { throw std::runtime_error("Too many sensors\n"); }
     const double n = static_cast<double>(measurement_steps_);
     const std::size_t steps_to_record =
@@ -670,11 +674,15 @@ void Model::addCell(const Triangle& triangle, std::size_t sensor_id, double spec
         const Triangle& a = cells_[inc].triangle();
         const Triangle& b = cells_[i].triangle();
         if (!triEq(a, b)) {
-            // Cell::validate, cell.cpp:21-25
-            if (triIntersects(a, b)) { throw std::runtime_error("Incoming triangle intersects an existing triangle\n"); }
-            if (triContainsPoint(a, b.p1) || triContainsPoint(a, b.p2) || triContainsPoint(a, b.p3)
-                || triContainsPoint(b, a.p1) || triContainsPoint(b, a.p2) || triContainsPoint(b, a.p3)) {
-                throw std::runtime_error("A triangle is contained within another triangle\n");
+            // Cell::validate, cell.cpp:21-25, with IntersectError / OverlapError (cell.cpp:150-164);
+            // IntersectError(existing = this cell, incoming = the other one) prints them that way.
+            const std::string ta = fmtTriangle(a.p1, a.p2, a.p3), tb = fmtTriangle(b.p1, b.p2, b.p3);
+            if (triIntersects(a, b)) { throw std::runtime_error("Incoming " + tb + "\nintersects\nExisting" + ta + '\n'); }
+            if (triContainsPoint(a, b.p1) || triContainsPoint(a, b.p2) || triContainsPoint(a, b.p3)) {
+                throw std::runtime_error(tb + "\nis contained within\n" + ta + '\n');
+            }
+            if (triContainsPoint(b, a.p1) || triContainsPoint(b, a.p2) || triContainsPoint(b, a.p3)) {
+                throw std::runtime_error(ta + "\nis contained within\n" + tb + '\n');
             }
             // Cell::findTransitionSurface, cell.cpp:81-98
             const std::array<Line, 3> l_inc = { cells_[inc].boundaries()[0].line, cells_[inc].boundaries()[1].line,
@@ -693,14 +701,10 @@ void Model::addCell(const Triangle& triangle, std::size_t sensor_id, double spec
             };
             for (const auto& l1 : l_inc) {
                 for (const auto& l2 : l_oth) {
-                    const Line* l = nullptr;
                     if (lineContains(l1, l2)) {
-                        l = &l2;
+                        if (!set_transition(inc, l2, i) || !set_transition(i, l2, inc)) { surfaceConflict(l1, l2); }
                     } else if (lineContains(l2, l1)) {
-                        l = &l1;
-                    }
-                    if (l && (!set_transition(inc, *l, i) || !set_transition(i, *l, inc))) {
-                        throw std::runtime_error("An existing surface conflicts with the location of the incoming surface.\n");
+                        if (!set_transition(inc, l1, i) || !set_transition(i, l1, inc)) { surfaceConflict(l2, l1); }
                     }
                 }
             }

@@ -13,6 +13,7 @@ For each workload it stores
                    mean and sd of t_steady / x_flux / y_flux, the per-block
                    (step_interval) temperature series for transient runs, the
                    budget's energy per phonon and the wall times;
+  ref_errors.json  the reference's error message for each broken model of error_cases.py;
   ref_<name>_state.npz  the reference's material tables at the workload's
                    temperatures, its temperature grid, one run's raw tallies
                    with the SensorInterpreter results computed from them, and
@@ -72,6 +73,28 @@ def block_series(final_temps: list, interval: int) -> list:
     return out
 
 
+def write_error_fixture() -> None:
+    """ref_errors.json: the message the reference gives for each broken model (newlines as spaces,
+    double quotes as single quotes, as ref_driver.cpp writes them)."""
+    sys.path.insert(0, HERE)
+    import error_cases
+
+    out = {}
+    with tempfile.TemporaryDirectory() as wd:
+        for name, model in error_cases.cases().items():
+            txt = os.path.join(wd, "m.txt")
+            res = os.path.join(wd, "m.json")
+            with open(txt, "w") as f:
+                f.write(refmodel.json_to_text(model))
+            subprocess.run([REF_BIN, txt, res], stdout=subprocess.DEVNULL)
+            with open(res) as f:
+                r = json.load(f)
+            assert not r["ok"], name
+            out[name] = r["error"]
+    with open(os.path.join(HERE, "ref_errors.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--runs", type=int, default=16)
@@ -79,6 +102,10 @@ def main() -> None:
     ap.add_argument("--only", default=None)
     args = ap.parse_args()
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    if args.only in (None, "errors"):
+        write_error_fixture()
+        if args.only == "errors":
+            return
     for name, (model, temps) in workloads().items():
         if args.only and name != args.only:
             continue
