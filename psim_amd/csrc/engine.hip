@@ -9,10 +9,14 @@
 //   tallies    counts [S][R] (i64) and flux [S][R][2] (f64), as Sensor::inc_energy_/inc_flux_.
 //   partials   per transport block: [W][S] count (i32) and flux (2 x f64) of one launch window.
 //
-// A run is: emission kernel (every phonon's birth state), then ceil(N / W) transport launches
-// each advancing every live phonon across W measurement steps with its state in registers,
-// each followed (when the window holds recorded steps) by a reduction of the block partials
-// into the tallies.
+//   lists      phonon ids sorted by birth step (counting sort) and two live lists: each
+//              transport launch reads the survivors of the previous window plus the phonons born
+//              in its own window and appends its survivors to the other list.
+//
+// A run is: emission kernel (every phonon's birth state and birth step), birth-step scan and
+// scatter, then ceil(N / W) transport launches each advancing the listed phonons across W
+// measurement steps with their state in registers, each followed (when the window holds
+// recorded steps) by a reduction of the block partials into the tallies.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,7 +35,7 @@ namespace {
 
 constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
-constexpr int LDS_TALLY_BYTES = 32 * 1024;  // budget for one block's [W][S] window tally
+constexpr int LDS_TALLY_BYTES = 48 * 1024;  // budget for one block's [W][S] window tally
 
 struct Tally {
     int32_t* __restrict__ pcnt;  // [grid][W*S]
@@ -64,9 +68,11 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 // Phonon builders, psim/src/phononBuilder.cpp:6-49, with EmitSurface::getPhononTime
 // (surface.cpp:67-69) and Line/Triangle::getRandPoint (geometry.cpp:140-143, 234-242).
 __global__ void __launch_bounds__(BLOCK) emit_kernel(Model m, Pool pool, int64_t total, uint64_t seed,
-    unsigned long long* __restrict__ counters, Fate fate) {
+    unsigned long long* __restrict__ counters, Fate fate, int32_t* __restrict__ birth_key,
+    uint32_t* __restrict__ birth_hist) {
     const int64_t id = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     uint32_t not_started = 0;
+    int32_t key = -1;
     if (id < total) {
         int lo = 0, hi = m.n_sources - 1;
         while (lo < hi) {
@@ -120,7 +126,10 @@ __global__ void __launch_bounds__(BLOCK) emit_kernel(Model m, Pool pool, int64_t
             cell = -1;
             not_started = 1;
             fate_write(fate, id, step, FATE_NOT_STARTED, p.px, p.py);
+        } else {
+            key = (int32_t)step;
         }
+        birth_key[id] = key;
         pool.px[id] = p.px;
         pool.py[id] = p.py;
         pool.dx[id] = p.dx;
@@ -132,15 +141,94 @@ __global__ void __launch_bounds__(BLOCK) emit_kernel(Model m, Pool pool, int64_t
         pool.cell[id] = cell;
         pool.meta[id] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u);
     }
+    // birth-step histogram; most waves of a cell-origin source share one key (step 0)
+    const int lane = threadIdx.x % WAVE;
+    const int key0 = __shfl(key, 0, WAVE);
+    const uint64_t same = __ballot(key == key0);
+    if (same == ~0ull) {
+        if (lane == 0 && key0 >= 0) { atomicAdd(&birth_hist[key0], (uint32_t)WAVE); }
+    } else if (key >= 0) {
+        atomicAdd(&birth_hist[key], 1u);
+    }
     const uint32_t ns = wave_sum(not_started);
-    if ((threadIdx.x % WAVE) == 0 && ns) { atomicAdd(&counters[PSIM_CNT_NOT_STARTED], (unsigned long long)ns); }
+    if (lane == 0 && ns) { atomicAdd(&counters[PSIM_CNT_NOT_STARTED], (unsigned long long)ns); }
+}
+
+// Exclusive scan of the birth-step histogram (one block): offset[k] = phonons born before
+// step k, offset[N] = all that start; cursor starts as a copy of offset.
+__global__ void __launch_bounds__(1024) birth_scan_kernel(const uint32_t* __restrict__ hist, int64_t N,
+    uint32_t* __restrict__ offset, uint32_t* __restrict__ cursor) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (N + 1023) / 1024;
+    const int64_t lo = t * per, hi = std::min<int64_t>(N, lo + per);
+    uint32_t sum = 0;
+    for (int64_t k = lo; k < hi; ++k) { sum += hist[k]; }
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan of the chunk sums
+        const uint32_t v = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (int64_t k = lo; k < hi; ++k) {
+        offset[k] = run;
+        cursor[k] = run;
+        run += hist[k];
+    }
+    if (t == 1023) { offset[N] = part[1023]; }
+}
+
+// Places every phonon id in born_order by birth step (order inside one step is arbitrary).
+__global__ void __launch_bounds__(BLOCK) birth_scatter_kernel(const int32_t* __restrict__ birth_key, int64_t total,
+    uint32_t* __restrict__ cursor, uint32_t* __restrict__ born_order) {
+    const int64_t id = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const int32_t key = (id < total) ? birth_key[id] : -1;
+    const int lane = threadIdx.x % WAVE;
+    const int key0 = __shfl(key, 0, WAVE);
+    const uint64_t same = __ballot(key == key0);
+    if (same == ~0ull) {
+        uint32_t base = 0;
+        if (lane == 0 && key0 >= 0) { base = atomicAdd(&cursor[key0], (uint32_t)WAVE); }
+        base = __shfl(base, 0, WAVE);
+        if (key0 >= 0) { born_order[base + lane] = (uint32_t)id; }
+    } else if (key >= 0) {
+        born_order[atomicAdd(&cursor[key], 1u)] = (uint32_t)id;
+    }
 }
 
 // Advances every live phonon of the pool across measurement steps [k0, k1): the body of
 // ModelSimulator::simulatePhonon (psim/src/modelSimulator.cpp:139-198) with the lane's state
 // in registers, pausing right after the measurement that brings its step to k1.
-__global__ void __launch_bounds__(BLOCK) transport_kernel(Model m, Pool pool, int64_t n_slots, int64_t k0,
-    int64_t k1, uint64_t seed, Tally tally, unsigned long long* __restrict__ counters, Fate fate) {
+struct Lists {
+    const uint32_t* __restrict__ live_in;  // phonons alive at step k0 (started)
+    uint32_t* __restrict__ live_out;  // survivors at step k1
+    uint32_t* __restrict__ n_live;  // [windows + 1]: n_live[w] entries of live_in, n_live[w + 1] appended
+    uint32_t* __restrict__ queue;  // [windows]: work-queue cursor of launch w
+    const uint32_t* __restrict__ born_order;  // ids sorted by birth step
+    const uint32_t* __restrict__ born_offset;  // [N + 1]
+    int32_t window;
+};
+
+#ifndef PSIM_TRANSPORT_WAVES_PER_EU
+#define PSIM_TRANSPORT_WAVES_PER_EU 2
+#endif
+
+// Lane states of the flattened event loop.
+enum : int { PH_NONE = 0, PH_START = 1, PH_DRIFT = 2 };
+
+// Advances the listed phonons across measurement steps [k0, k1).  Each lane runs
+// ModelSimulator::simulatePhonon (psim/src/modelSimulator.cpp:139-198) for one phonon at a
+// time, pausing right after the measurement that brings its step to k1, and then takes the
+// next phonon from the launch's work queue.  The loop is flattened so that every iteration of
+// every lane performs exactly one nextImpact (the inner handleImpacts loop, :205-225, becomes
+// PH_DRIFT iterations): lanes do not wait for siblings with more collisions or longer
+// lifetimes.  The floating-point sequence per phonon is unchanged.
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PSIM_TRANSPORT_WAVES_PER_EU)))
+transport_kernel(Model m, Pool pool, Lists lists, int64_t k0, int64_t k1, uint64_t seed, Tally tally,
+    unsigned long long* __restrict__ counters, Fate fate) {
     extern __shared__ unsigned char lds_raw[];
     const int S = m.n_sensors;
     const int WS = tally.rec ? tally.W * S : 0;
@@ -155,58 +243,114 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Model m, Pool pool, in
     __syncthreads();
 
     LaneCounters c = { 0, 0, 0, 0, 0, 0, 0, 0 };
-    const int64_t stride = (int64_t)gridDim.x * BLOCK;
-    for (int64_t slot = (int64_t)blockIdx.x * BLOCK + threadIdx.x; slot < n_slots; slot += stride) {
-        const int32_t cell0 = pool.cell[slot];
-        if (cell0 < 0) { continue; }
-        const uint32_t meta = pool.meta[slot];
-        double age = pool.age[slot];
-        int64_t step;
-        if (meta & META_STARTED) {
-            step = k0;
-        } else {
-            step = (int64_t)(age / m.step_time);
-            if (step >= k1) { continue; }
+    const int lane = threadIdx.x % WAVE;
+    const int64_t n_live = lists.n_live[lists.window];
+    const int64_t b0 = lists.born_offset[k0];
+    const int64_t n_items = n_live + (int64_t)lists.born_offset[k1] - b0;
+    uint32_t* __restrict__ n_out = lists.n_live + lists.window + 1;
+    uint32_t* __restrict__ queue = lists.queue + lists.window;
+
+    int phase = PH_NONE;
+    uint32_t slot = 0;
+    Ph p;
+    p.px = p.py = p.dx = p.dy = p.freq = p.vel = 0.;
+    p.cell = p.bin = p.pol = 0;
+    p.sign = 1;
+    p.step = 0;
+    double age = 0., tts = 0., ttm = 0.;
+    double relax[3] = { 0., 0., 0. };
+    int rate_sensor = 0;
+    int events = 0;
+    double drift_time = 0., drifted = 0.;
+    int sensor0 = 0, collisions = 0;
+    psim_stream st;
+    psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, 0, 0);
+
+    for (;;) {
+        if (phase == PH_NONE) {  // take the next phonon of the queue: one atomic per wave
+            const uint64_t need = __ballot(1);
+            const int leader = __ffsll((unsigned long long)need) - 1;
+            uint32_t base = 0;
+            if (lane == leader) { base = atomicAdd(queue, (uint32_t)__popcll(need)); }
+            base = __shfl(base, leader, WAVE);
+            const int64_t item = (int64_t)base + __popcll(need & ((1ull << lane) - 1ull));
+            if (item >= n_items) { break; }
+            const bool started = item < n_live;
+            slot = started ? lists.live_in[item] : lists.born_order[b0 + (item - n_live)];
+            const uint32_t meta = pool.meta[slot];
+            p.cell = pool.cell[slot];
+            p.px = pool.px[slot];
+            p.py = pool.py[slot];
+            p.dx = pool.dx[slot];
+            p.dy = pool.dy[slot];
+            p.freq = pool.freq[slot];
+            p.vel = pool.vel[slot];
+            p.bin = (int)(meta & META_BIN_MASK);
+            p.pol = (meta & META_TA) ? POL_TA : POL_LA;
+            p.sign = (meta & META_NEG) ? -1 : 1;
+            age = pool.age[slot];
+            p.step = started ? k0 : (int64_t)(age / m.step_time);
+            tts = pool.tts[slot];
+            rate_sensor = (int)(meta >> META_RATE_SHIFT);
+            if (tts > 0.) { relax_rates(m, rate_sensor, p, relax); }
+            ttm = 0.;
+            events = 0;
+            psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, slot, (uint32_t)p.step);
+            phase = PH_START;
         }
-        Ph p;
-        p.cell = cell0;
-        p.px = pool.px[slot];
-        p.py = pool.py[slot];
-        p.dx = pool.dx[slot];
-        p.dy = pool.dy[slot];
-        p.freq = pool.freq[slot];
-        p.bin = (int)(meta & META_BIN_MASK);
-        p.pol = (meta & META_TA) ? POL_TA : POL_LA;
-        p.sign = (meta & META_NEG) ? -1 : 1;
-        p.step = step;
-        p.vel = pool.vel[slot];
-        double tts = pool.tts[slot];
-        int rate_sensor = (int)(meta >> META_RATE_SHIFT);
-        double relax[3] = { 0., 0., 0. };
-        if (tts > 0.) { relax_rates(m, rate_sensor, p, relax); }
-        double ttm = 0.;
-        psim_stream st;
-        psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, (uint32_t)slot, (uint32_t)step);
-        int fate_cause = 0;
-        int events = 0;
-        for (;;) {
+        int done = 0;  // 1: paused at k1, otherwise a FATE_* cause
+        if (phase == PH_START) {  // top of simulatePhonon's loop
             if (++events > PSIM_MAX_EVENTS_PER_STEP) {
                 c.stalled++;
-                fate_cause = FATE_STALLED;
-                break;
+                done = FATE_STALLED + 1;
+            } else {
+                if (tts <= 0.) {
+                    rate_sensor = m.cells[p.cell].sensor;
+                    relax_rates(m, rate_sensor, p, relax);
+                    tts = SCALING_FACTOR * -psim_log(psim_urand(&st)) / (0. + relax[0] + relax[1] + relax[2]);
+                }
+                if (ttm <= 0.) { ttm = (double)(p.step + 1) * m.sim_time / (double)m.N - age; }
+                drift_time = (ttm < tts) ? ttm : tts;
+                sensor0 = m.cells[p.cell].sensor;
+                drifted = 0.;
+                collisions = 0;
+                phase = PH_DRIFT;
             }
-            if (tts <= 0.) {
-                rate_sensor = m.cells[p.cell].sensor;
-                relax_rates(m, rate_sensor, p, relax);
-                tts = SCALING_FACTOR * -psim_log(psim_urand(&st)) / (0. + relax[0] + relax[1] + relax[2]);
+        }
+        if (phase == PH_DRIFT && !done) {  // one nextImpact of handleImpacts
+            double t = 0.;
+            const bool hit = next_impact(m, p, drift_time - drifted, t, st, c);
+            bool drift_over = !hit;
+            double result = drifted;
+            if (hit) {
+                if (p.cell < 0) {
+                    c.absorbed++;
+                    done = FATE_ABSORBED + 1;
+                } else {
+                    drifted += t;
+                    if (++collisions > MAX_COLLISIONS) {  // Phonon::setRandPoint -> Triangle::getRandPoint
+                        double r1 = psim_urand(&st);
+                        double r2 = psim_urand(&st);
+                        const psim_cell& cl = m.cells[p.cell];
+                        if (r1 + r2 > 1.) {
+                            r1 = 1 - r1;
+                            r2 = 1 - r2;
+                        }
+                        p.px = cl.x1 + (cl.x2 - cl.x1) * r1 + (cl.x3 - cl.x1) * r2;
+                        p.py = cl.y1 + (cl.y2 - cl.y1) * r1 + (cl.y3 - cl.y1) * r2;
+                        c.relocations++;
+                        result = drift_time;
+                        drift_over = true;
+                    } else if (m.cells[p.cell].sensor != sensor0) {
+                        result = drifted;
+                        drift_over = true;
+                    }
+                }
             }
-            if (ttm <= 0.) { ttm = (double)(p.step + 1) * m.sim_time / (double)m.N - age; }
-            double drift_time = (ttm < tts) ? ttm : tts;
-            const int sensor = m.cells[p.cell].sensor;
-            double drifted;
-            if (handle_impacts(m, p, drift_time, sensor, drifted, st, c)) {
-                if (m.cells[p.cell].sensor != sensor) { drift_time = drifted; }
-                const double factor = p.vel * (drift_time - drifted);  // Phonon::drift
+            if (drift_over && !done) {  // rest of simulatePhonon's loop body
+                phase = PH_START;
+                if (m.cells[p.cell].sensor != sensor0) { drift_time = result; }
+                const double factor = p.vel * (drift_time - result);  // Phonon::drift
                 p.px += p.dx * factor;
                 p.py += p.dy * factor;
                 age += drift_time;
@@ -229,11 +373,10 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Model m, Pool pool, in
                                 c.tallies++;
                             }
                         }
-                        if (p.step == k1) { break; }
+                        if (p.step == k1) { done = 1; }
                     } else {
                         c.expired++;
-                        fate_cause = FATE_EXPIRED;
-                        break;
+                        done = FATE_EXPIRED + 1;
                     }
                 } else if (!m.phasor && tts == 0.) {
                     c.scatters++;
@@ -241,27 +384,38 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Model m, Pool pool, in
                 } else {
                     tts = 0.;
                 }
-            } else {
-                c.absorbed++;
-                fate_cause = FATE_ABSORBED;
-                break;
             }
         }
-        if (fate_cause) {
-            pool.cell[slot] = -1;
-            fate_write(fate, slot, fate_cause == FATE_EXPIRED ? p.step - 1 : p.step, fate_cause, p.px, p.py);
-        } else {
-            pool.px[slot] = p.px;
-            pool.py[slot] = p.py;
-            pool.dx[slot] = p.dx;
-            pool.dy[slot] = p.dy;
-            pool.freq[slot] = p.freq;
-            pool.vel[slot] = p.vel;
-            pool.age[slot] = age;
-            pool.tts[slot] = tts;
-            pool.cell[slot] = p.cell;
-            pool.meta[slot] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
-                              | META_STARTED | ((uint32_t)rate_sensor << META_RATE_SHIFT);
+        // Phonons that paused or died: survivors are appended to the next window's list (one
+        // atomic per wave), the dead leave the pool.
+        const bool survive = done == 1;
+        const uint64_t mask = __ballot(survive);
+        if (mask) {
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t base = 0;
+            if (lane == leader) { base = atomicAdd(n_out, (uint32_t)__popcll(mask)); }
+            base = __shfl(base, leader, WAVE);
+            if (survive) { lists.live_out[base + __popcll(mask & ((1ull << lane) - 1ull))] = slot; }
+        }
+        if (done) {
+            if (survive) {
+                pool.px[slot] = p.px;
+                pool.py[slot] = p.py;
+                pool.dx[slot] = p.dx;
+                pool.dy[slot] = p.dy;
+                pool.freq[slot] = p.freq;
+                pool.vel[slot] = p.vel;
+                pool.age[slot] = age;
+                pool.tts[slot] = tts;
+                pool.cell[slot] = p.cell;
+                pool.meta[slot] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
+                                  | META_STARTED | ((uint32_t)rate_sensor << META_RATE_SHIFT);
+            } else {
+                const int cause = done - 1;
+                pool.cell[slot] = -1;
+                fate_write(fate, slot, cause == FATE_EXPIRED ? p.step - 1 : p.step, cause, p.px, p.py);
+            }
+            phase = PH_NONE;
         }
     }
 
@@ -271,8 +425,8 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Model m, Pool pool, in
     const int slots[8] = { PSIM_CNT_PHONON_STEPS, PSIM_CNT_SCATTERS, PSIM_CNT_IMPACTS, PSIM_CNT_RELOCATIONS,
         PSIM_CNT_ABSORBED, PSIM_CNT_EXPIRED, PSIM_CNT_TALLIES, PSIM_CNT_STALLED };
     for (int i = 0; i < 8; ++i) {
-        const uint32_t s = wave_sum(v[i]);
-        if ((threadIdx.x % WAVE) == 0 && s) { atomicAdd(&counters[slots[i]], (unsigned long long)s); }
+        const uint32_t sum = wave_sum(v[i]);
+        if (lane == 0 && sum) { atomicAdd(&counters[slots[i]], (unsigned long long)sum); }
     }
     if (WS) {
         __syncthreads();
@@ -344,6 +498,17 @@ struct psim_gpu_ctx {
     bool fate_enabled = false;
     Fate fate{ nullptr, nullptr, nullptr };
     int64_t fate_cap = 0;
+    // birth ordering and live lists
+    int32_t* d_birth_key = nullptr;
+    uint32_t* d_born_order = nullptr;
+    uint32_t* d_live[2] = { nullptr, nullptr };
+    int64_t list_cap = 0;
+    uint32_t* d_birth_hist = nullptr;
+    uint32_t* d_born_offset = nullptr;
+    uint32_t* d_cursor = nullptr;
+    uint32_t* d_n_live = nullptr;
+    uint32_t* d_queue = nullptr;
+    int64_t steps_cap = 0;
     int32_t window = 0;
     // timing of the last run
     double step_ms = 0., run_ms = 0.;
@@ -389,6 +554,26 @@ void free_fate(psim_gpu_ctx* c) {
     c->fate_cap = 0;
 }
 
+void free_lists(psim_gpu_ctx* c) {
+    void* ps[] = { c->d_birth_key, c->d_born_order, c->d_live[0], c->d_live[1] };
+    for (void* p : ps) {
+        if (p) { (void)hipFree(p); }
+    }
+    c->d_birth_key = nullptr;
+    c->d_born_order = nullptr;
+    c->d_live[0] = c->d_live[1] = nullptr;
+    c->list_cap = 0;
+}
+
+void free_steps(psim_gpu_ctx* c) {
+    void* ps[] = { c->d_birth_hist, c->d_born_offset, c->d_cursor, c->d_n_live, c->d_queue };
+    for (void* p : ps) {
+        if (p) { (void)hipFree(p); }
+    }
+    c->d_birth_hist = c->d_born_offset = c->d_cursor = c->d_n_live = c->d_queue = nullptr;
+    c->steps_cap = 0;
+}
+
 void free_partials(psim_gpu_ctx* c) {
     if (c->d_pcnt) { (void)hipFree(c->d_pcnt); }
     if (c->d_pfx) { (void)hipFree(c->d_pfx); }
@@ -409,12 +594,40 @@ int upload_array(psim_gpu_ctx* c, const T* host, size_t n, const T** out) {
     return PSIM_OK;
 }
 
-// Measurement steps per transport launch: the LDS tally of a recording window holds
-// W x S entries of 20 bytes.
-int32_t window_for(const psim_gpu_ctx* c) {
-    if (c->window > 0) { return c->window; }
+// Measurement steps per transport launch.  A recording window (one that reaches a tallied step)
+// keeps W x S tally entries of 20 bytes in LDS; windows before the first recorded step carry no
+// tally, so by default they are merged into one launch (lanes refill from the work queue, so
+// phonon lifetimes of any length balance within it; each launch ends with a tail in which the
+// last phonons finish, which is why fewer launches run faster).  An explicit window length
+// applies to both kinds.
+constexpr int32_t DEFAULT_FREE_WINDOW = 1 << 30;
+
+int32_t record_window_for(const psim_gpu_ctx* c) {
     const int64_t by_lds = std::max<int64_t>(1, LDS_TALLY_BYTES / (20 * std::max(1, c->n_sensors)));
+    if (c->window > 0) { return (int32_t)std::min<int64_t>(c->window, std::max<int64_t>(by_lds, 1)); }
     return (int32_t)std::min<int64_t>(64, by_lds);
+}
+
+int32_t free_window_for(const psim_gpu_ctx* c) { return c->window > 0 ? c->window : DEFAULT_FREE_WINDOW; }
+
+// Launch windows [k0, k1) over N steps: step values k0+1..k1 are reached inside window [k0, k1);
+// windows whose step values all lie below adj (the first recorded step) are tally-free.
+std::vector<std::pair<int64_t, int64_t>> plan_windows(int64_t N, int64_t adj, int32_t w_free, int32_t w_rec) {
+    std::vector<std::pair<int64_t, int64_t>> out;
+    int64_t k0 = 0;
+    while (k0 < N) {
+        int64_t k1;
+        if (k0 + w_free <= adj - 1) {
+            k1 = k0 + w_free;
+        } else if (k0 < adj - 1) {
+            k1 = adj - 1;
+        } else {
+            k1 = std::min<int64_t>(N, k0 + w_rec);
+        }
+        out.emplace_back(k0, k1);
+        k0 = k1;
+    }
+    return out;
 }
 
 int run_impl(psim_gpu_ctx* c, uint64_t seed) {
@@ -457,8 +670,38 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
         }
         c->fate_cap = (int64_t)cap;
     }
+    if (c->list_cap < n) {
+        free_lists(c);
+        const size_t cap = (size_t)std::max<int64_t>(n, 1);
+        c->d_birth_key = dalloc<int32_t>(cap);
+        c->d_born_order = dalloc<uint32_t>(cap);
+        c->d_live[0] = dalloc<uint32_t>(cap);
+        c->d_live[1] = dalloc<uint32_t>(cap);
+        if (!c->d_birth_key || !c->d_born_order || !c->d_live[0] || !c->d_live[1]) {
+            free_lists(c);
+            return fail(c, PSIM_E_DEVICE, "hipMalloc failed for the phonon lists");
+        }
+        c->list_cap = (int64_t)cap;
+    }
     const Fate fate = c->fate_enabled ? c->fate : Fate{ nullptr, nullptr, nullptr };
-    const int32_t W = window_for(c);
+    const int32_t W = record_window_for(c);
+    const int64_t N = P.num_measurements;
+    const auto windows = plan_windows(N, P.step_adjustment, free_window_for(c), W);
+    const int64_t n_windows = (int64_t)windows.size();
+    const int64_t steps_need = std::max<int64_t>(N + 1, n_windows + 1);
+    if (c->steps_cap < steps_need) {
+        free_steps(c);
+        c->d_birth_hist = dalloc<uint32_t>((size_t)steps_need);
+        c->d_born_offset = dalloc<uint32_t>((size_t)steps_need);
+        c->d_cursor = dalloc<uint32_t>((size_t)steps_need);
+        c->d_n_live = dalloc<uint32_t>((size_t)steps_need);
+        c->d_queue = dalloc<uint32_t>((size_t)steps_need);
+        if (!c->d_birth_hist || !c->d_born_offset || !c->d_cursor || !c->d_n_live || !c->d_queue) {
+            free_steps(c);
+            return fail(c, PSIM_E_DEVICE, "hipMalloc failed for the step arrays");
+        }
+        c->steps_cap = steps_need;
+    }
     const int64_t blocks_needed = (n + BLOCK - 1) / BLOCK;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks_needed, (int64_t)c->n_cu * 8));
     const int64_t WS = (int64_t)W * S;
@@ -477,24 +720,34 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
     HIP_CHECK(c, hipMemsetAsync(c->d_counts, 0, sizeof(int64_t) * (size_t)std::max<int64_t>(1, S * R), c->stream));
     HIP_CHECK(c, hipMemsetAsync(c->d_flux, 0, sizeof(double) * (size_t)std::max<int64_t>(1, 2 * S * R), c->stream));
     HIP_CHECK(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * PSIM_NUM_COUNTERS, c->stream));
+    HIP_CHECK(c, hipMemsetAsync(c->d_birth_hist, 0, sizeof(uint32_t) * (size_t)steps_need, c->stream));
+    HIP_CHECK(c, hipMemsetAsync(c->d_n_live, 0, sizeof(uint32_t) * (size_t)steps_need, c->stream));
+    HIP_CHECK(c, hipMemsetAsync(c->d_queue, 0, sizeof(uint32_t) * (size_t)steps_need, c->stream));
     HIP_CHECK(c, hipEventRecord(c->ev[0], c->stream));
     if (n > 0) {
         hipLaunchKernelGGL(emit_kernel, dim3((unsigned)blocks_needed), dim3(BLOCK), 0, c->stream, c->m, c->pool, n,
-            seed, c->d_counters, fate);
+            seed, c->d_counters, fate, c->d_birth_key, c->d_birth_hist);
+        HIP_CHECK(c, hipGetLastError());
+        hipLaunchKernelGGL(birth_scan_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_birth_hist, N, c->d_born_offset,
+            c->d_cursor);
+        HIP_CHECK(c, hipGetLastError());
+        hipLaunchKernelGGL(birth_scatter_kernel, dim3((unsigned)blocks_needed), dim3(BLOCK), 0, c->stream,
+            c->d_birth_key, n, c->d_cursor, c->d_born_order);
         HIP_CHECK(c, hipGetLastError());
     }
     c->step_ms = 0.;
     c->step_launches = 0;
-    const int64_t N = P.num_measurements;
-    for (int64_t k0 = 0; k0 < N && n > 0; k0 += W) {
-        const int64_t k1 = std::min<int64_t>(N, k0 + W);
+    for (int32_t w = 0; w < (int32_t)n_windows && n > 0; ++w) {
+        const int64_t k0 = windows[(size_t)w].first, k1 = windows[(size_t)w].second;
+        const Lists lists{ c->d_live[w % 2], c->d_live[(w + 1) % 2], c->d_n_live, c->d_queue, c->d_born_order,
+            c->d_born_offset, w };
         // window steps k0+1 .. k1 map to tally indices (k - adj) in [0, R)
-        const bool rec = (k1 >= P.step_adjustment) && (k0 + 1 - P.step_adjustment < R);
+        const bool rec = (k1 >= P.step_adjustment) && (k0 + 1 - P.step_adjustment < R) && (k1 - k0 <= W);
         Tally t{ c->d_pcnt, c->d_pfx, c->d_pfy, rec ? 1 : 0, W };
         const size_t lds = rec ? (size_t)WS * 20 : 0;
         HIP_CHECK(c, hipEventRecord(c->ev[1], c->stream));
-        hipLaunchKernelGGL(transport_kernel, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool, n, k0, k1, seed,
-            t, c->d_counters, fate);
+        hipLaunchKernelGGL(transport_kernel, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool, lists, k0, k1,
+            seed, t, c->d_counters, fate);
         HIP_CHECK(c, hipGetLastError());
         HIP_CHECK(c, hipEventRecord(c->ev[2], c->stream));
         if (rec) {
@@ -566,6 +819,8 @@ void psim_gpu_destroy(psim_gpu_ctx* c) {
     free_pool(c);
     free_fate(c);
     free_partials(c);
+    free_lists(c);
+    free_steps(c);
     if (c->d_counts) { (void)hipFree(c->d_counts); }
     if (c->d_flux) { (void)hipFree(c->d_flux); }
     if (c->d_counters) { (void)hipFree(c->d_counters); }
