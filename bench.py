@@ -140,6 +140,14 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    # HBM traffic of the transport kernel per launch, measured by rocprofv3 --pmc FETCH_SIZE and
+    # WRITE_SIZE (scaled by the copy-kernel calibration) on this workload; see profiles/.
+    traffic_file = os.path.join(ROOT, "profiles", "transport_kernel_lshape_1e7_traffic.json")
+    if args.phonons == 10_000_000 and os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        line["roofline"]["traffic"] = tr["bytes_per_dispatch"]
+        line["roofline"]["traffic_bytes_per_phonon_step"] = tr["bytes_per_phonon_step"]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_phonons)
     print(json.dumps(line), flush=True)
