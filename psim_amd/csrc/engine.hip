@@ -216,18 +216,20 @@ struct Lists {
 #define PSIM_TRANSPORT_WAVES_PER_EU 2
 #endif
 
-// Lane states of the flattened event loop.
-enum : int { PH_NONE = 0, PH_START = 1, PH_DRIFT = 2 };
+// Lane states of the scheduled event loop.
+enum : int { S_FETCH = 0, S_TTS = 1, S_NI = 2, S_COLL = 3, S_SCAT = 4, S_EXIT = 5, S_PAUSED = 6, S_DEAD = 7 };
 
-// Advances the listed phonons across measurement steps [k0, k1).  Each lane runs
-// ModelSimulator::simulatePhonon (psim/src/modelSimulator.cpp:139-198) for one phonon at a
-// time, pausing right after the measurement that brings its step to k1, and then takes the
-// next phonon from the launch's work queue.  The loop is flattened so that every iteration of
-// every lane performs exactly one nextImpact (the inner handleImpacts loop, :205-225, becomes
-// PH_DRIFT iterations): lanes do not wait for siblings with more collisions or longer
-// lifetimes.  The floating-point sequence per phonon is unchanged.
+// Advances the listed phonons across measurement steps [k0, k1).  Each lane carries one phonon
+// through ModelSimulator::simulatePhonon (psim/src/modelSimulator.cpp:139-198), pausing right
+// after the measurement that brings its step to k1, then takes the next phonon of the launch's
+// work queue.  The loop is split into stages -- fetch, draw a scattering time, one nextImpact,
+// surface handling after a hit, scatter -- and on every iteration the wave runs the single
+// stage that most of its lanes wait for: the expensive rare stages (scatter, collision, the
+// scattering-time draw) run once for many lanes instead of once per lane that needs them.
+// Each lane still performs exactly the reference's sequence of operations for its phonon, so
+// trajectories are unchanged.
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PSIM_TRANSPORT_WAVES_PER_EU)))
-transport_kernel(Model m, Pool pool, Lists lists, int64_t k0, int64_t k1, uint64_t seed, Tally tally,
+transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64_t seed, Tally tally,
     unsigned long long* __restrict__ counters, Fate fate) {
     extern __shared__ unsigned char lds_raw[];
     const int S = m.n_sensors;
@@ -244,13 +246,14 @@ transport_kernel(Model m, Pool pool, Lists lists, int64_t k0, int64_t k1, uint64
 
     LaneCounters c = { 0, 0, 0, 0, 0, 0, 0, 0 };
     const int lane = threadIdx.x % WAVE;
+    const uint64_t lane_below = (1ull << lane) - 1ull;
     const int64_t n_live = lists.n_live[lists.window];
     const int64_t b0 = lists.born_offset[k0];
     const int64_t n_items = n_live + (int64_t)lists.born_offset[k1] - b0;
     uint32_t* __restrict__ n_out = lists.n_live + lists.window + 1;
     uint32_t* __restrict__ queue = lists.queue + lists.window;
 
-    int phase = PH_NONE;
+    int state = S_FETCH;
     uint32_t slot = 0;
     Ph p;
     p.px = p.py = p.dx = p.dy = p.freq = p.vel = 0.;
@@ -261,73 +264,153 @@ transport_kernel(Model m, Pool pool, Lists lists, int64_t k0, int64_t k1, uint64
     double relax[3] = { 0., 0., 0. };
     int rate_sensor = 0;
     int events = 0;
-    double drift_time = 0., drifted = 0.;
+    double drift_time = 0., drifted = 0., t_hit = 0.;
     int sensor0 = 0, collisions = 0;
     psim_stream st;
     psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, 0, 0);
 
-    for (;;) {
-        if (phase == PH_NONE) {  // take the next phonon of the queue: one atomic per wave
-            const uint64_t need = __ballot(1);
-            const int leader = __ffsll((unsigned long long)need) - 1;
-            uint32_t base = 0;
-            if (lane == leader) { base = atomicAdd(queue, (uint32_t)__popcll(need)); }
-            base = __shfl(base, leader, WAVE);
-            const int64_t item = (int64_t)base + __popcll(need & ((1ull << lane) - 1ull));
-            if (item >= n_items) { break; }
-            const bool started = item < n_live;
-            slot = started ? lists.live_in[item] : lists.born_order[b0 + (item - n_live)];
-            const uint32_t meta = pool.meta[slot];
-            p.cell = pool.cell[slot];
-            p.px = pool.px[slot];
-            p.py = pool.py[slot];
-            p.dx = pool.dx[slot];
-            p.dy = pool.dy[slot];
-            p.freq = pool.freq[slot];
-            p.vel = pool.vel[slot];
-            p.bin = (int)(meta & META_BIN_MASK);
-            p.pol = (meta & META_TA) ? POL_TA : POL_LA;
-            p.sign = (meta & META_NEG) ? -1 : 1;
-            age = pool.age[slot];
-            p.step = started ? k0 : (int64_t)(age / m.step_time);
-            tts = pool.tts[slot];
-            rate_sensor = (int)(meta >> META_RATE_SHIFT);
-            if (tts > 0.) { relax_rates(m, rate_sensor, p, relax); }
-            ttm = 0.;
+    // Top of simulatePhonon's loop up to the nextImpact call (modelSimulator.cpp:154-167): the
+    // stall bound, S_TTS when a scattering time is due (drawn once, then drift_setup), else
+    // drift_setup directly.
+    auto drift_setup = [&]() {
+        if (ttm <= 0.) { ttm = (double)(p.step + 1) * m.sim_time / (double)m.N - age; }
+        drift_time = (ttm < tts) ? ttm : tts;
+        sensor0 = m.cells[p.cell].sensor;
+        drifted = 0.;
+        collisions = 0;
+        state = S_NI;
+    };
+    auto begin_event = [&]() {
+        if (++events > PSIM_MAX_EVENTS_PER_STEP) {
+            c.stalled++;
+            state = S_DEAD + FATE_STALLED * 8;
+            return;
+        }
+        if (tts <= 0.) {
+            state = S_TTS;
+            return;
+        }
+        drift_setup();
+    };
+    // The rest of simulatePhonon's loop body once handleImpacts returned `result`.
+    auto end_drift = [&](double result) {
+        if (m.cells[p.cell].sensor != sensor0) { drift_time = result; }
+        const double factor = p.vel * (drift_time - result);  // Phonon::drift
+        p.px += p.dx * factor;
+        p.py += p.dy * factor;
+        age += drift_time;
+        ttm -= drift_time;
+        tts -= drift_time;
+        if (ttm == 0.) {
+            c.steps++;
             events = 0;
-            psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, slot, (uint32_t)p.step);
-            phase = PH_START;
-        }
-        int done = 0;  // 1: paused at k1, otherwise a FATE_* cause
-        if (phase == PH_START) {  // top of simulatePhonon's loop
-            if (++events > PSIM_MAX_EVENTS_PER_STEP) {
-                c.stalled++;
-                done = FATE_STALLED + 1;
-            } else {
-                if (tts <= 0.) {
-                    rate_sensor = m.cells[p.cell].sensor;
-                    relax_rates(m, rate_sensor, p, relax);
-                    tts = SCALING_FACTOR * -psim_log(psim_urand(&st)) / (0. + relax[0] + relax[1] + relax[2]);
+            if (++p.step < m.N) {
+                psim_stream_set_step(&st, (uint32_t)p.step);
+                if (p.step >= m.adj) {  // Sensor::updateHeatParams, sensor.cpp:43-52
+                    const int idx = p.step - m.adj;
+                    if (idx < m.R && tally.rec) {
+                        const int s = m.cells[p.cell].sensor;
+                        const int e = (p.step - k0 - 1) * S + s;
+                        const double sg = (double)p.sign;
+                        atomicAdd(&lcnt[e], p.sign);
+                        atomicAdd(&lfx[e], p.dx * p.vel * sg);
+                        atomicAdd(&lfy[e], p.dy * p.vel * sg);
+                        c.tallies++;
+                    }
                 }
-                if (ttm <= 0.) { ttm = (double)(p.step + 1) * m.sim_time / (double)m.N - age; }
-                drift_time = (ttm < tts) ? ttm : tts;
-                sensor0 = m.cells[p.cell].sensor;
-                drifted = 0.;
-                collisions = 0;
-                phase = PH_DRIFT;
+                if (p.step == k1) {
+                    state = S_PAUSED;
+                    return;
+                }
+            } else {
+                c.expired++;
+                state = S_DEAD + FATE_EXPIRED * 8;
+                return;
             }
+        } else if (!m.phasor && tts == 0.) {
+            state = S_SCAT;
+            return;
+        } else {
+            tts = 0.;
         }
-        if (phase == PH_DRIFT && !done) {  // one nextImpact of handleImpacts
-            double t = 0.;
-            const bool hit = next_impact(m, p, drift_time - drifted, t, st, c);
-            bool drift_over = !hit;
-            double result = drifted;
-            if (hit) {
+        begin_event();
+    };
+
+    for (;;) {
+        // Pick the stage with the most waiting lanes (wave-uniform).
+        const uint64_t m_fetch = __ballot(state == S_FETCH);
+        const uint64_t m_tts = __ballot(state == S_TTS);
+        const uint64_t m_ni = __ballot(state == S_NI);
+        const uint64_t m_coll = __ballot(state == S_COLL);
+        const uint64_t m_scat = __ballot(state == S_SCAT);
+        int sel = S_EXIT;
+        int best = 0;
+        const int n_fetch = __popcll(m_fetch), n_tts = __popcll(m_tts), n_ni = __popcll(m_ni);
+        const int n_coll = __popcll(m_coll), n_scat = __popcll(m_scat);
+        if (n_ni > best) { best = n_ni; sel = S_NI; }
+        if (n_tts > best) { best = n_tts; sel = S_TTS; }
+        if (n_scat > best) { best = n_scat; sel = S_SCAT; }
+        if (n_coll > best) { best = n_coll; sel = S_COLL; }
+        if (n_fetch > best) { best = n_fetch; sel = S_FETCH; }
+        if (sel == S_EXIT) { break; }  // every lane has exited
+
+        if (sel == S_FETCH) {
+            if (state == S_FETCH) {  // next phonon of the queue: one atomic per wave
+                const int leader = __ffsll((unsigned long long)m_fetch) - 1;
+                uint32_t base = 0;
+                if (lane == leader) { base = atomicAdd(queue, (uint32_t)n_fetch); }
+                base = __shfl(base, leader, WAVE);
+                const int64_t item = (int64_t)base + __popcll(m_fetch & lane_below);
+                if (item >= n_items) {
+                    state = S_EXIT;
+                } else {
+                    const bool started = item < n_live;
+                    slot = started ? lists.live_in[item] : lists.born_order[b0 + (item - n_live)];
+                    const uint32_t meta = pool.meta[slot];
+                    p.cell = pool.cell[slot];
+                    p.px = pool.px[slot];
+                    p.py = pool.py[slot];
+                    p.dx = pool.dx[slot];
+                    p.dy = pool.dy[slot];
+                    p.freq = pool.freq[slot];
+                    p.vel = pool.vel[slot];
+                    p.bin = (int)(meta & META_BIN_MASK);
+                    p.pol = (meta & META_TA) ? POL_TA : POL_LA;
+                    p.sign = (meta & META_NEG) ? -1 : 1;
+                    age = pool.age[slot];
+                    p.step = started ? k0 : (int32_t)(age / m.step_time);
+                    tts = pool.tts[slot];
+                    rate_sensor = (int)(meta >> META_RATE_SHIFT);
+                    if (tts > 0.) { relax_rates(m, rate_sensor, p, relax); }
+                    ttm = 0.;
+                    events = 0;
+                    psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, slot, (uint32_t)p.step);
+                    begin_event();
+                }
+            }
+        } else if (sel == S_TTS) {
+            if (state == S_TTS) {
+                rate_sensor = m.cells[p.cell].sensor;
+                relax_rates(m, rate_sensor, p, relax);
+                tts = SCALING_FACTOR * -psim_log(psim_urand(&st)) / (0. + relax[0] + relax[1] + relax[2]);
+                drift_setup();
+            }
+        } else if (sel == S_NI) {
+            if (state == S_NI) {  // one nextImpact of handleImpacts
+                if (find_impact(m, p, drift_time - drifted, t_hit, c)) {
+                    state = S_COLL;
+                } else {
+                    end_drift(drifted);
+                }
+            }
+        } else if (sel == S_COLL) {
+            if (state == S_COLL) {  // surface handling, then handleImpacts' bookkeeping
+                handle_surface_collision(m, p, p.px, p.py, st);
                 if (p.cell < 0) {
                     c.absorbed++;
-                    done = FATE_ABSORBED + 1;
+                    state = S_DEAD + FATE_ABSORBED * 8;
                 } else {
-                    drifted += t;
+                    drifted += t_hit;
                     if (++collisions > MAX_COLLISIONS) {  // Phonon::setRandPoint -> Triangle::getRandPoint
                         double r1 = psim_urand(&st);
                         double r2 = psim_urand(&st);
@@ -339,83 +422,51 @@ transport_kernel(Model m, Pool pool, Lists lists, int64_t k0, int64_t k1, uint64
                         p.px = cl.x1 + (cl.x2 - cl.x1) * r1 + (cl.x3 - cl.x1) * r2;
                         p.py = cl.y1 + (cl.y2 - cl.y1) * r1 + (cl.y3 - cl.y1) * r2;
                         c.relocations++;
-                        result = drift_time;
-                        drift_over = true;
+                        end_drift(drift_time);
                     } else if (m.cells[p.cell].sensor != sensor0) {
-                        result = drifted;
-                        drift_over = true;
-                    }
-                }
-            }
-            if (drift_over && !done) {  // rest of simulatePhonon's loop body
-                phase = PH_START;
-                if (m.cells[p.cell].sensor != sensor0) { drift_time = result; }
-                const double factor = p.vel * (drift_time - result);  // Phonon::drift
-                p.px += p.dx * factor;
-                p.py += p.dy * factor;
-                age += drift_time;
-                ttm -= drift_time;
-                tts -= drift_time;
-                if (ttm == 0.) {
-                    c.steps++;
-                    events = 0;
-                    if (++p.step < m.N) {
-                        psim_stream_set_step(&st, (uint32_t)p.step);
-                        if (p.step >= m.adj) {  // Sensor::updateHeatParams, sensor.cpp:43-52
-                            const int64_t idx = p.step - m.adj;
-                            if (idx < m.R && tally.rec) {
-                                const int s = m.cells[p.cell].sensor;
-                                const int e = (int)(p.step - k0 - 1) * S + s;
-                                const double sg = (double)p.sign;
-                                atomicAdd(&lcnt[e], p.sign);
-                                atomicAdd(&lfx[e], p.dx * p.vel * sg);
-                                atomicAdd(&lfy[e], p.dy * p.vel * sg);
-                                c.tallies++;
-                            }
-                        }
-                        if (p.step == k1) { done = 1; }
+                        end_drift(drifted);
                     } else {
-                        c.expired++;
-                        done = FATE_EXPIRED + 1;
+                        state = S_NI;
                     }
-                } else if (!m.phasor && tts == 0.) {
-                    c.scatters++;
-                    scatter(m, p, relax, st);
-                } else {
-                    tts = 0.;
                 }
             }
-        }
-        // Phonons that paused or died: survivors are appended to the next window's list (one
-        // atomic per wave), the dead leave the pool.
-        const bool survive = done == 1;
-        const uint64_t mask = __ballot(survive);
-        if (mask) {
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            uint32_t base = 0;
-            if (lane == leader) { base = atomicAdd(n_out, (uint32_t)__popcll(mask)); }
-            base = __shfl(base, leader, WAVE);
-            if (survive) { lists.live_out[base + __popcll(mask & ((1ull << lane) - 1ull))] = slot; }
-        }
-        if (done) {
-            if (survive) {
-                pool.px[slot] = p.px;
-                pool.py[slot] = p.py;
-                pool.dx[slot] = p.dx;
-                pool.dy[slot] = p.dy;
-                pool.freq[slot] = p.freq;
-                pool.vel[slot] = p.vel;
-                pool.age[slot] = age;
-                pool.tts[slot] = tts;
-                pool.cell[slot] = p.cell;
-                pool.meta[slot] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
-                                  | META_STARTED | ((uint32_t)rate_sensor << META_RATE_SHIFT);
-            } else {
-                const int cause = done - 1;
-                pool.cell[slot] = -1;
-                fate_write(fate, slot, cause == FATE_EXPIRED ? p.step - 1 : p.step, cause, p.px, p.py);
+        } else {  // S_SCAT
+            if (state == S_SCAT) {
+                c.scatters++;
+                scatter(m, p, relax, st);
+                begin_event();
             }
-            phase = PH_NONE;
+        }
+
+        // Phonons that paused or died this iteration: survivors are appended to the next
+        // window's list (one atomic per wave), the dead leave the pool.
+        const bool survive = state == S_PAUSED;
+        const uint64_t m_surv = __ballot(survive);
+        if (m_surv) {
+            const int leader = __ffsll((unsigned long long)m_surv) - 1;
+            uint32_t base = 0;
+            if (lane == leader) { base = atomicAdd(n_out, (uint32_t)__popcll(m_surv)); }
+            base = __shfl(base, leader, WAVE);
+            if (survive) { lists.live_out[base + __popcll(m_surv & lane_below)] = slot; }
+        }
+        if (survive) {
+            pool.px[slot] = p.px;
+            pool.py[slot] = p.py;
+            pool.dx[slot] = p.dx;
+            pool.dy[slot] = p.dy;
+            pool.freq[slot] = p.freq;
+            pool.vel[slot] = p.vel;
+            pool.age[slot] = age;
+            pool.tts[slot] = tts;
+            pool.cell[slot] = p.cell;
+            pool.meta[slot] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
+                              | META_STARTED | ((uint32_t)rate_sensor << META_RATE_SHIFT);
+            state = S_FETCH;
+        } else if (state >= S_DEAD) {
+            const int cause = (state - S_DEAD) / 8;
+            pool.cell[slot] = -1;
+            fate_write(fate, slot, cause == FATE_EXPIRED ? p.step - 1 : p.step, cause, p.px, p.py);
+            state = S_FETCH;
         }
     }
 
@@ -746,8 +797,8 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
         Tally t{ c->d_pcnt, c->d_pfx, c->d_pfy, rec ? 1 : 0, W };
         const size_t lds = rec ? (size_t)WS * 20 : 0;
         HIP_CHECK(c, hipEventRecord(c->ev[1], c->stream));
-        hipLaunchKernelGGL(transport_kernel, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool, lists, k0, k1,
-            seed, t, c->d_counters, fate);
+        hipLaunchKernelGGL(transport_kernel, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool, lists, (int32_t)k0,
+            (int32_t)k1, seed, t, c->d_counters, fate);
         HIP_CHECK(c, hipGetLastError());
         HIP_CHECK(c, hipEventRecord(c->ev[2], c->stream));
         if (rec) {
@@ -845,7 +896,9 @@ int psim_gpu_upload(psim_gpu_ctx* c, const psim_geom* g, const psim_tables* t, c
         return fail(c, PSIM_E_INVALID, "tables incomplete");
     }
     if (s->n_sources < 0 || (s->n_sources > 0 && !s->sources)) { return fail(c, PSIM_E_INVALID, "bad sources"); }
-    if (p->num_measurements <= 0 || p->num_record < 0 || p->step_time <= 0. || p->total_phonons < 0
+    if (p->num_measurements <= 0 || p->num_measurements >= INT32_MAX || p->num_record < 0
+        || p->num_record > p->num_measurements || p->step_adjustment < 0 || p->step_adjustment > p->num_measurements
+        || p->step_time <= 0. || p->total_phonons < 0
         || p->total_phonons > (int64_t)UINT32_MAX) {
         return fail(c, PSIM_E_INVALID, "bad run parameters");
     }
@@ -919,9 +972,9 @@ int psim_gpu_upload(psim_gpu_ctx* c, const psim_geom* g, const psim_tables* t, c
     m.n_sensors = g->n_sensors;
     m.full = p->full;
     m.phasor = p->phasor;
-    m.N = p->num_measurements;
-    m.adj = p->step_adjustment;
-    m.R = p->num_record;
+    m.N = (int32_t)p->num_measurements;
+    m.adj = (int32_t)p->step_adjustment;
+    m.R = (int32_t)p->num_record;
     m.sim_time = p->sim_time;
     m.step_time = p->step_time;
     m.t_eq = p->t_eq;

@@ -16,8 +16,18 @@
 #include <stdint.h>
 
 #include "psim_gpu.h"
+#if defined(PSIM_OUTLINE_PHILOX) && PSIM_OUTLINE_PHILOX
+#define PSIM_PHILOX_QUAL __host__ __device__ static __attribute__((noinline))
+#endif
 #include "psim_math.h"
 #include "psim_rng.h"
+
+// Rare, long stages may be kept out of line to shorten register live ranges in the event loop.
+#if defined(PSIM_OUTLINE_STAGES) && PSIM_OUTLINE_STAGES
+#define PSIM_STAGE __device__ __noinline__
+#else
+#define PSIM_STAGE __device__ __forceinline__
+#endif
 
 namespace psim_dev {
 
@@ -56,9 +66,9 @@ struct Model {
     int32_t n_sensors;
     int32_t full;
     int32_t phasor;
-    int64_t N;
-    int64_t adj;
-    int64_t R;
+    int32_t N;  // measurement steps (validated to fit int32 at upload)
+    int32_t adj;
+    int32_t R;
     double sim_time;
     double step_time;
     double t_eq;
@@ -80,7 +90,7 @@ struct Pool {
 struct Ph {
     double px, py, dx, dy, freq, vel;
     int32_t cell, bin, pol, sign;
-    int64_t step;
+    int32_t step;
 };
 
 struct Line {
@@ -245,7 +255,7 @@ __device__ __forceinline__ void boundary_handle(double nx, double ny, double spe
 }
 
 // Cell::handleSurfaceCollision -> CompositeSurface::handlePhonon and the surface handlers.
-__device__ void handle_surface_collision(const Model& m, Ph& p, double poix, double poiy, psim_stream& st) {
+PSIM_STAGE void handle_surface_collision(const Model& m, Ph& p, double poix, double poiy, psim_stream& st) {
     const psim_edge* e3 = m.edges + 3 * p.cell;
     for (int b = 0; b < 3; ++b) {
         const psim_edge& e = e3[b];
@@ -283,6 +293,44 @@ __device__ void handle_surface_collision(const Model& m, Ph& p, double poix, dou
 struct LaneCounters {
     uint32_t steps, scatters, impacts, relocations, absorbed, expired, tallies, stalled;
 };
+
+// The geometric part of ModelSimulator::nextImpact (modelSimulator.cpp:87-117): the nearest edge
+// crossing of the path p -> p + v * time.  On a hit the phonon is moved to the point of impact
+// and the impact time is returned in out_time; the collision itself is handled separately
+// (modelSimulator.cpp:118).
+__device__ bool find_impact(const Model& m, Ph& p, double time, double& out_time, LaneCounters& c) {
+    const double vx = p.dx * p.vel;
+    const double vy = p.dy * p.vel;
+    const double sx = p.px, sy = p.py;
+    const double ex = p.px + time * vx, ey = p.py + time * vy;
+    if (pt_eq(sx, sy, ex, ey)) { return false; }
+    const Line path = make_line(sx, sy, ex, ey);
+    const psim_edge* e3 = m.edges + 3 * p.cell;
+    bool hit = false;
+    double ix = 0., iy = 0.;
+    for (int b = 0; b < 3; ++b) {
+        const Line l = edge_line(e3[b]);
+        double qx, qy;
+        if (get_intersection(l, path, qx, qy) && !pt_eq(qx, qy, sx, sy)) {
+            const double tx = (vx > VELOCITY_EPS || vx < -VELOCITY_EPS) ? (qx - sx) / vx : time;
+            const double ty = (vy > VELOCITY_EPS || vy < -VELOCITY_EPS) ? (qy - sy) / vy : time;
+            const double it = (tx <= ty) ? tx : ty;
+            if (it <= time) {
+                time = it;
+                ix = qx;
+                iy = qy;
+                hit = true;
+            }
+        }
+    }
+    if (hit) {
+        p.px = ix;
+        p.py = iy;
+        c.impacts++;
+        out_time = time;
+    }
+    return hit;
+}
 
 // ModelSimulator::nextImpact, modelSimulator.cpp:87-122
 __device__ bool next_impact(const Model& m, Ph& p, double time, double& out_time, psim_stream& st,
@@ -372,7 +420,7 @@ __device__ __forceinline__ void relax_rates(const Model& m, int sensor, const Ph
 }
 
 // ModelSimulator::scatter, modelSimulator.cpp:124-137
-__device__ __forceinline__ void scatter(const Model& m, Ph& p, const double* r, psim_stream& st) {
+PSIM_STAGE void scatter(const Model& m, Ph& p, const double* r, psim_stream& st) {
     const double tau_inv = 0. + r[0] + r[1] + r[2];
     const double rand = psim_urand(&st);
     if (rand <= (r[0] + r[1]) / tau_inv) {
