@@ -936,7 +936,7 @@ int or_prepare(or_model* m, uint64_t seed) {
         s->t4 = pow(s->t_init, 4);
     }
     /* Model::getTotalInitialEnergy (transform_reduce, seq), model.cpp:196-201 */
-    double* per_cell = (double*)malloc(sizeof(double) * (size_t)m->n_cells);
+    double* per_cell = (double*)calloc((size_t)m->n_cells, sizeof(double));
     for (int ci = 0; ci < m->n_cells; ++ci) {
         per_cell[ci] = cell_init_energy(m, ci, m->t_eq) + cell_emit_energy(m, ci, m->t_eq);
     }
@@ -947,11 +947,11 @@ int or_prepare(or_model* m, uint64_t seed) {
     /* Sources in the reference's builder order: per cell, the cell-origin builder, then every
      * emit sub-surface of its three boundaries. */
     free(m->sources);
-    int cap = m->n_cells;
+    size_t cap = (size_t)m->n_cells;
     for (int ci = 0; ci < m->n_cells; ++ci) {
-        for (int b = 0; b < 3; ++b) { cap += m->cells[ci].b[b].n_emit; }
+        for (int b = 0; b < 3; ++b) { cap += (size_t)m->cells[ci].b[b].n_emit; }
     }
-    m->sources = (source_t*)calloc((size_t)cap, sizeof(source_t));
+    m->sources = (source_t*)calloc(cap, sizeof(source_t));
     m->n_sources = 0;
     int64_t offset = 0;
     uint32_t sidx = 0;
@@ -1504,7 +1504,7 @@ static double post_run_eff_energy(or_model* m, const double* final_temp) {
         saved[si] = m->sensors[si].t_steady;
         m->sensors[si].t_steady = final_temp[si];
     }
-    double* per_cell = (double*)malloc(sizeof(double) * (size_t)m->n_cells);
+    double* per_cell = (double*)calloc((size_t)m->n_cells, sizeof(double));
     for (int ci = 0; ci < m->n_cells; ++ci) {
         per_cell[ci] = cell_init_energy(m, ci, m->t_eq) + cell_emit_energy(m, ci, m->t_eq);
     }

@@ -266,6 +266,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
     int events = 0;
     double drift_time = 0., drifted = 0., t_hit = 0.;
     int sensor0 = 0, collisions = 0;
+    int cur_sensor = 0;  // sensor of p.cell
     psim_stream st;
     psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, 0, 0);
 
@@ -275,7 +276,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
     auto drift_setup = [&]() {
         if (ttm <= 0.) { ttm = (double)(p.step + 1) * m.sim_time / (double)m.N - age; }
         drift_time = (ttm < tts) ? ttm : tts;
-        sensor0 = m.cells[p.cell].sensor;
+        sensor0 = cur_sensor;
         drifted = 0.;
         collisions = 0;
         state = S_NI;
@@ -294,7 +295,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
     };
     // The rest of simulatePhonon's loop body once handleImpacts returned `result`.
     auto end_drift = [&](double result) {
-        if (m.cells[p.cell].sensor != sensor0) { drift_time = result; }
+        if (cur_sensor != sensor0) { drift_time = result; }
         const double factor = p.vel * (drift_time - result);  // Phonon::drift
         p.px += p.dx * factor;
         p.py += p.dy * factor;
@@ -309,8 +310,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
                 if (p.step >= m.adj) {  // Sensor::updateHeatParams, sensor.cpp:43-52
                     const int idx = p.step - m.adj;
                     if (idx < m.R && tally.rec) {
-                        const int s = m.cells[p.cell].sensor;
-                        const int e = (p.step - k0 - 1) * S + s;
+                        const int e = (p.step - k0 - 1) * S + cur_sensor;
                         const double sg = (double)p.sign;
                         atomicAdd(&lcnt[e], p.sign);
                         atomicAdd(&lfx[e], p.dx * p.vel * sg);
@@ -337,18 +337,17 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
     };
 
     for (;;) {
-        // Pick the stage with the most waiting lanes (wave-uniform).
+        // Pick the stage with the most waiting lanes (wave-uniform).  A scatter is always
+        // followed by a scattering-time draw, so the two share one stage.
         const uint64_t m_fetch = __ballot(state == S_FETCH);
-        const uint64_t m_tts = __ballot(state == S_TTS);
         const uint64_t m_ni = __ballot(state == S_NI);
         const uint64_t m_coll = __ballot(state == S_COLL);
-        const uint64_t m_scat = __ballot(state == S_SCAT);
+        const uint64_t m_scat = __ballot(state == S_SCAT || state == S_TTS);
         int sel = S_EXIT;
         int best = 0;
-        const int n_fetch = __popcll(m_fetch), n_tts = __popcll(m_tts), n_ni = __popcll(m_ni);
+        const int n_fetch = __popcll(m_fetch), n_ni = __popcll(m_ni);
         const int n_coll = __popcll(m_coll), n_scat = __popcll(m_scat);
         if (n_ni > best) { best = n_ni; sel = S_NI; }
-        if (n_tts > best) { best = n_tts; sel = S_TTS; }
         if (n_scat > best) { best = n_scat; sel = S_SCAT; }
         if (n_coll > best) { best = n_coll; sel = S_COLL; }
         if (n_fetch > best) { best = n_fetch; sel = S_FETCH; }
@@ -381,19 +380,13 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
                     p.step = started ? k0 : (int32_t)(age / m.step_time);
                     tts = pool.tts[slot];
                     rate_sensor = (int)(meta >> META_RATE_SHIFT);
+                    cur_sensor = m.cells[p.cell].sensor;
                     if (tts > 0.) { relax_rates(m, rate_sensor, p, relax); }
                     ttm = 0.;
                     events = 0;
                     psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, slot, (uint32_t)p.step);
                     begin_event();
                 }
-            }
-        } else if (sel == S_TTS) {
-            if (state == S_TTS) {
-                rate_sensor = m.cells[p.cell].sensor;
-                relax_rates(m, rate_sensor, p, relax);
-                tts = SCALING_FACTOR * -psim_log(psim_urand(&st)) / (0. + relax[0] + relax[1] + relax[2]);
-                drift_setup();
             }
         } else if (sel == S_NI) {
             if (state == S_NI) {  // one nextImpact of handleImpacts
@@ -410,6 +403,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
                     c.absorbed++;
                     state = S_DEAD + FATE_ABSORBED * 8;
                 } else {
+                    cur_sensor = m.cells[p.cell].sensor;
                     drifted += t_hit;
                     if (++collisions > MAX_COLLISIONS) {  // Phonon::setRandPoint -> Triangle::getRandPoint
                         double r1 = psim_urand(&st);
@@ -423,18 +417,29 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
                         p.py = cl.y1 + (cl.y2 - cl.y1) * r1 + (cl.y3 - cl.y1) * r2;
                         c.relocations++;
                         end_drift(drift_time);
-                    } else if (m.cells[p.cell].sensor != sensor0) {
+                    } else if (cur_sensor != sensor0) {
                         end_drift(drifted);
                     } else {
                         state = S_NI;
                     }
                 }
             }
-        } else {  // S_SCAT
+        } else {  // S_SCAT: scatter (lanes in S_SCAT), then the scattering-time draw
             if (state == S_SCAT) {
                 c.scatters++;
                 scatter(m, p, relax, st);
-                begin_event();
+                if (++events > PSIM_MAX_EVENTS_PER_STEP) {  // begin_event's stall bound
+                    c.stalled++;
+                    state = S_DEAD + FATE_STALLED * 8;
+                } else {
+                    state = S_TTS;  // tts is 0 here, so begin_event would ask for a draw
+                }
+            }
+            if (state == S_TTS) {
+                rate_sensor = cur_sensor;
+                relax_rates(m, rate_sensor, p, relax);
+                tts = SCALING_FACTOR * -psim_log(psim_urand(&st)) / (0. + relax[0] + relax[1] + relax[2]);
+                drift_setup();
             }
         }
 
@@ -967,6 +972,27 @@ int psim_gpu_upload(psim_gpu_ctx* c, const psim_geom* g, const psim_tables* t, c
     PSIM_UPLOAD(cdf, t->cdf, nb * t->n_tables);
     PSIM_UPLOAD(frac, t->la_frac, nb * t->n_tables);
     PSIM_UPLOAD(sources, s->sources, s->n_sources);
+    // Guide tables of the frequency search, when every cumulative table is non-decreasing (a
+    // NaN fails the test too).
+    bool monotone = true;
+    for (int64_t i = 0; i + 1 < (int64_t)nb * t->n_tables && monotone; ++i) {
+        if ((i + 1) % (int64_t)nb != 0 && !(t->cdf[i] <= t->cdf[i + 1])) { monotone = false; }
+    }
+    if (monotone) {
+        std::vector<uint16_t> guide((size_t)(GUIDE + 1) * t->n_tables);
+        for (int k = 0; k < t->n_tables; ++k) {
+            const double* cdf = t->cdf + (size_t)k * nb;
+            int i = 0;
+            for (int g = 0; g <= GUIDE; ++g) {  // first index whose value exceeds g / GUIDE
+                const double x = (double)g / (double)GUIDE;
+                while (i < (int)nb && !(cdf[i] > x)) { ++i; }
+                guide[(size_t)k * (GUIDE + 1) + g] = (uint16_t)i;
+            }
+        }
+        PSIM_UPLOAD(guide, guide.data(), guide.size());
+    } else {
+        m.guide = nullptr;
+    }
 #undef PSIM_UPLOAD
     m.n_sources = s->n_sources;
     m.n_sensors = g->n_sensors;

@@ -16,18 +16,8 @@
 #include <stdint.h>
 
 #include "psim_gpu.h"
-#if defined(PSIM_OUTLINE_PHILOX) && PSIM_OUTLINE_PHILOX
-#define PSIM_PHILOX_QUAL __host__ __device__ static __attribute__((noinline))
-#endif
 #include "psim_math.h"
 #include "psim_rng.h"
-
-// Rare, long stages may be kept out of line to shorten register live ranges in the event loop.
-#if defined(PSIM_OUTLINE_STAGES) && PSIM_OUTLINE_STAGES
-#define PSIM_STAGE __device__ __noinline__
-#else
-#define PSIM_STAGE __device__ __forceinline__
-#endif
 
 namespace psim_dev {
 
@@ -38,6 +28,7 @@ constexpr int MAX_COLLISIONS = 100;  // modelSimulator.cpp:25
 constexpr double HBAR = 1.054517e-34;  // material.cpp:12
 constexpr double BOLTZ = 1.38065e-23;  // material.cpp:13
 constexpr int NBINS = PSIM_NUM_FREQ_BINS;
+constexpr int GUIDE = 2048;  // guide-table cells per frequency table (a power of two)
 
 enum : int { POL_LA = 0, POL_TA = 1 };
 enum : int { FATE_ABSORBED = 1, FATE_EXPIRED = 2, FATE_NOT_STARTED = 3, FATE_STALLED = 4 };
@@ -61,6 +52,9 @@ struct Model {
     const double* __restrict__ vel_ta;
     const double* __restrict__ cdf;
     const double* __restrict__ frac;
+    // [n_tables][GUIDE + 1] start indices of the frequency search, or null when some table is not
+    // monotone (then the bisection runs as written).
+    const uint16_t* __restrict__ guide;
     const psim_source* __restrict__ sources;
     int32_t n_sources;
     int32_t n_sensors;
@@ -193,19 +187,35 @@ __device__ __forceinline__ bool get_intersection(const Line& s, const Line& o, d
 
 // ---- phonon updates -------------------------------------------------------------------
 
-// Material::freqIndex bisection returning `high`, material.cpp:64-75
+// Material::freqIndex (material.cpp:64-75): a bisection over the cumulative table that keeps
+// cdf[low] <= rand < cdf[high] and returns `high`.  For a non-decreasing table that is the first
+// index whose value exceeds rand, clamped to [1, NBINS - 1] (bin 0 folds into bin 1, as in the
+// reference).  With a guide table (guide[g] = first index whose value exceeds g / GUIDE, for
+// g = 0..GUIDE) that index lies in [guide[g], guide[g + 1]] for g = floor(rand * GUIDE), which a
+// short scan finds: one dependent load in place of ten.  Tables that are not non-decreasing
+// have no guide and run the bisection.
 __device__ __forceinline__ void freq_index(const double* __restrict__ cdf, const double* __restrict__ frac,
-    psim_stream& st, int& bin, int& pol) {
-    int low = 0, high = NBINS - 1;
-    int mid = low + (high - low) / 2;
+    const uint16_t* __restrict__ guide, psim_stream& st, int& bin, int& pol) {
     const double rand = psim_urand(&st);
-    while (high - low > 1) {
-        if (rand < cdf[mid]) {
-            high = mid;
-        } else {
-            low = mid;
+    int high;
+    if (guide) {
+        const int g = (int)(rand * (double)GUIDE);
+        int i = guide[g];
+        const int end = guide[g + 1];
+        while (i < end && !(rand < cdf[i])) { ++i; }
+        high = i < 1 ? 1 : (i > NBINS - 1 ? NBINS - 1 : i);
+    } else {
+        int low = 0;
+        high = NBINS - 1;
+        int mid = low + (high - low) / 2;
+        while (high - low > 1) {
+            if (rand < cdf[mid]) {
+                high = mid;
+            } else {
+                low = mid;
+            }
+            mid = (low + high) / 2;
         }
-        mid = (low + high) / 2;
     }
     bin = high;
     pol = (psim_urand(&st) <= frac[high]) ? POL_LA : POL_TA;
@@ -219,7 +229,8 @@ __device__ __forceinline__ double bin_velocity(const Model& m, int mat, int bin,
 // material.cpp:77-80
 __device__ __forceinline__ void resample(const Model& m, int mat, int table, Ph& p, psim_stream& st) {
     int bin, pol;
-    freq_index(m.cdf + (int64_t)table * NBINS, m.frac + (int64_t)table * NBINS, st, bin, pol);
+    freq_index(m.cdf + (int64_t)table * NBINS, m.frac + (int64_t)table * NBINS,
+        m.guide ? m.guide + (int64_t)table * (GUIDE + 1) : nullptr, st, bin, pol);
     const psim_material& mt = m.mats[mat];
     const double f0 = m.freq[mat * NBINS + bin];
     p.freq = m.full ? f0 : f0 + (2. * psim_urand(&st) - 1.) * mt.freq_width / 2.;
@@ -255,7 +266,7 @@ __device__ __forceinline__ void boundary_handle(double nx, double ny, double spe
 }
 
 // Cell::handleSurfaceCollision -> CompositeSurface::handlePhonon and the surface handlers.
-PSIM_STAGE void handle_surface_collision(const Model& m, Ph& p, double poix, double poiy, psim_stream& st) {
+__device__ __forceinline__ void handle_surface_collision(const Model& m, Ph& p, double poix, double poiy, psim_stream& st) {
     const psim_edge* e3 = m.edges + 3 * p.cell;
     for (int b = 0; b < 3; ++b) {
         const psim_edge& e = e3[b];
@@ -332,78 +343,6 @@ __device__ bool find_impact(const Model& m, Ph& p, double time, double& out_time
     return hit;
 }
 
-// ModelSimulator::nextImpact, modelSimulator.cpp:87-122
-__device__ bool next_impact(const Model& m, Ph& p, double time, double& out_time, psim_stream& st,
-    LaneCounters& c) {
-    const double vx = p.dx * p.vel;
-    const double vy = p.dy * p.vel;
-    const double sx = p.px, sy = p.py;
-    const double ex = p.px + time * vx, ey = p.py + time * vy;
-    if (pt_eq(sx, sy, ex, ey)) { return false; }
-    const Line path = make_line(sx, sy, ex, ey);
-    const psim_edge* e3 = m.edges + 3 * p.cell;
-    bool hit = false;
-    double ix = 0., iy = 0.;
-    for (int b = 0; b < 3; ++b) {
-        const Line l = edge_line(e3[b]);
-        double qx, qy;
-        if (get_intersection(l, path, qx, qy) && !pt_eq(qx, qy, sx, sy)) {
-            const double tx = (vx > VELOCITY_EPS || vx < -VELOCITY_EPS) ? (qx - sx) / vx : time;
-            const double ty = (vy > VELOCITY_EPS || vy < -VELOCITY_EPS) ? (qy - sy) / vy : time;
-            const double it = (tx <= ty) ? tx : ty;
-            if (it <= time) {
-                time = it;
-                ix = qx;
-                iy = qy;
-                hit = true;
-            }
-        }
-    }
-    if (hit) {
-        p.px = ix;
-        p.py = iy;
-        c.impacts++;
-        handle_surface_collision(m, p, ix, iy, st);
-        out_time = time;
-    }
-    return hit;
-}
-
-// ModelSimulator::handleImpacts, modelSimulator.cpp:205-225; false = the phonon left the system.
-__device__ bool handle_impacts(const Model& m, Ph& p, double drift_time, int sensor, double& drifted_out,
-    psim_stream& st, LaneCounters& c) {
-    double t = 0.;
-    bool hit = next_impact(m, p, drift_time, t, st, c);
-    double drifted = 0.;
-    int collisions = 0;
-    while (hit) {
-        if (p.cell < 0) { return false; }
-        drifted += t;
-        if (++collisions > MAX_COLLISIONS) {  // Phonon::setRandPoint -> Triangle::getRandPoint
-            double r1 = psim_urand(&st);
-            double r2 = psim_urand(&st);
-            const psim_cell& cl = m.cells[p.cell];
-            if (r1 + r2 > 1.) {
-                r1 = 1 - r1;
-                r2 = 1 - r2;
-            }
-            p.px = cl.x1 + (cl.x2 - cl.x1) * r1 + (cl.x3 - cl.x1) * r2;
-            p.py = cl.y1 + (cl.y2 - cl.y1) * r1 + (cl.y3 - cl.y1) * r2;
-            c.relocations++;
-            drifted_out = drift_time;
-            return true;
-        }
-        if (m.cells[p.cell].sensor != sensor) {
-            drifted_out = drifted;
-            return true;
-        }
-        hit = next_impact(m, p, drift_time - drifted, t, st, c);
-    }
-    if (p.cell < 0) { return false; }
-    drifted_out = drifted;
-    return true;
-}
-
 // Material::relaxRates at the sensor temperature, material.cpp:54-56, 207-239
 __device__ __forceinline__ void relax_rates(const Model& m, int sensor, const Ph& p, double* r) {
     const psim_sensor& s = m.sensors[sensor];
@@ -420,7 +359,7 @@ __device__ __forceinline__ void relax_rates(const Model& m, int sensor, const Ph
 }
 
 // ModelSimulator::scatter, modelSimulator.cpp:124-137
-PSIM_STAGE void scatter(const Model& m, Ph& p, const double* r, psim_stream& st) {
+__device__ __forceinline__ void scatter(const Model& m, Ph& p, const double* r, psim_stream& st) {
     const double tau_inv = 0. + r[0] + r[1] + r[2];
     const double rand = psim_urand(&st);
     if (rand <= (r[0] + r[1]) / tau_inv) {

@@ -30,10 +30,6 @@
 #else
 #define PSIM_QUAL static inline
 #endif
-/* Qualifier of the Philox block function; a GPU build may keep a single out-of-line copy. */
-#ifndef PSIM_PHILOX_QUAL
-#define PSIM_PHILOX_QUAL PSIM_QUAL
-#endif
 
 #define PSIM_TAG_TRANSPORT 0u
 #define PSIM_TAG_EMIT 1u
@@ -63,7 +59,7 @@ PSIM_QUAL psim_u32x4 psim_philox_round(psim_u32x4 c, uint32_t k0, uint32_t k1) {
 
 /* Philox4x32 with 10 rounds; the key is bumped by the Weyl constants between
  * rounds. */
-PSIM_PHILOX_QUAL psim_u32x4 psim_philox4x32_10(psim_u32x4 c, uint32_t k0, uint32_t k1) {
+PSIM_QUAL psim_u32x4 psim_philox4x32_10(psim_u32x4 c, uint32_t k0, uint32_t k1) {
     for (int i = 0; i < 9; ++i) {
         c = psim_philox_round(c, k0, k1);
         k0 += PSIM_PHILOX_W0;
