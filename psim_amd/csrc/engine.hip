@@ -4,14 +4,16 @@
 // Data layout in HBM:
 //   model      cells, edges (3 per cell), sub-surfaces, sensors, materials, frequency/velocity
 //              arrays and the CDF / LA-fraction tables: uploaded once, read-only, L2-resident.
-//   pool       one slot per phonon id, structure of arrays: px py dx dy freq vel age tts (f64),
-//              cell (i32, -1 = gone), meta (u32: bin, polarization, sign, started, rate sensor).
+//   pool       emission output, one slot per phonon id, structure of arrays: px py dx dy freq vel
+//              age tts (f64), cell (i32, -1 = never starts), meta (u32: bin, polarization, sign,
+//              started, rate sensor).  Read once, when the phonon's birth window fetches it.
 //   tallies    counts [S][R] (i64) and flux [S][R][2] (f64), as Sensor::inc_energy_/inc_flux_.
 //   partials   per transport block: [W][S] count (i32) and flux (2 x f64) of one launch window.
 //
-//   lists      phonon ids sorted by birth step (counting sort) and two live lists: each
-//              transport launch reads the survivors of the previous window plus the phonons born
-//              in its own window and appends its survivors to the other list.
+//   live       two dense buffers with the pool's fields plus the phonon id: each transport launch
+//              reads the survivors of the previous window from one, in order, plus the phonons
+//              born in its own window (ids sorted by birth step, counting sort), and appends its
+//              survivors to the other.
 //
 // A run is: emission kernel (every phonon's birth state and birth step), birth-step scan and
 // scatter, then ceil(N / W) transport launches each advancing the listed phonons across W
@@ -36,6 +38,7 @@ namespace {
 constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
 constexpr int LDS_TALLY_BYTES = 48 * 1024;  // budget for one block's [W][S] window tally
+constexpr uint32_t QUEUE_CLAIM = 32;  // work-queue items a wave claims per atomic
 
 struct Tally {
     int32_t* __restrict__ pcnt;  // [grid][W*S]
@@ -139,6 +142,7 @@ __global__ void __launch_bounds__(BLOCK) emit_kernel(Model m, Pool pool, int64_t
         pool.age[id] = birth;
         pool.tts[id] = 0.;
         pool.cell[id] = cell;
+        pool.sensor[id] = cl.sensor;
         pool.meta[id] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u);
     }
     // birth-step histogram; most waves of a cell-origin source share one key (step 0)
@@ -199,13 +203,34 @@ __global__ void __launch_bounds__(BLOCK) birth_scatter_kernel(const int32_t* __r
     }
 }
 
-// Advances every live phonon of the pool across measurement steps [k0, k1): the body of
-// ModelSimulator::simulatePhonon (psim/src/modelSimulator.cpp:139-198) with the lane's state
-// in registers, pausing right after the measurement that brings its step to k1.
+// State of the phonons that are alive between two launch windows, densely packed in the order
+// the previous launch paused them, with the phonon id (the key of its Philox stream).  A launch
+// reads its started phonons from one buffer in work-queue order and appends its survivors to the
+// other, so the phonons that neighbouring lanes and waves fetch or park share cache lines.
+struct Live {
+    double* __restrict__ px;
+    double* __restrict__ py;
+    double* __restrict__ dx;
+    double* __restrict__ dy;
+    double* __restrict__ freq;
+    double* __restrict__ vel;
+    double* __restrict__ age;
+    double* __restrict__ tts;
+    double* __restrict__ rate0;  // relaxation rates of the last scattering-time draw
+    double* __restrict__ rate1;
+    double* __restrict__ rate2;
+    int32_t* __restrict__ cell;
+    int32_t* __restrict__ sensor;  // sensor of the cell
+    uint32_t* __restrict__ meta;
+    uint32_t* __restrict__ id;
+};
+
+// Work of one transport launch: the started phonons (Live buffer `in`, n_live[window] entries)
+// followed by the phonons born in the window (born_order, read from the emission pool).
 struct Lists {
-    const uint32_t* __restrict__ live_in;  // phonons alive at step k0 (started)
-    uint32_t* __restrict__ live_out;  // survivors at step k1
-    uint32_t* __restrict__ n_live;  // [windows + 1]: n_live[w] entries of live_in, n_live[w + 1] appended
+    Live in;  // phonons alive at step k0 (started)
+    Live out;  // survivors at step k1
+    uint32_t* __restrict__ n_live;  // [windows + 1]: n_live[w] entries of `in`, n_live[w + 1] appended to `out`
     uint32_t* __restrict__ queue;  // [windows]: work-queue cursor of launch w
     const uint32_t* __restrict__ born_order;  // ids sorted by birth step
     const uint32_t* __restrict__ born_offset;  // [N + 1]
@@ -262,13 +287,15 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
     p.step = 0;
     double age = 0., tts = 0., ttm = 0.;
     double relax[3] = { 0., 0., 0. };
-    int rate_sensor = 0;
     int events = 0;
     double drift_time = 0., drifted = 0., t_hit = 0.;
     int sensor0 = 0, collisions = 0;
     int cur_sensor = 0;  // sensor of p.cell
     psim_stream st;
     psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, 0, 0);
+    // Work-queue items the wave has claimed but not handed out yet (wave-uniform): a refill takes
+    // them in order and claims QUEUE_CLAIM more with one atomic when they run out.
+    uint32_t res_next = 0, res_end = 0;
 
     // Top of simulatePhonon's loop up to the nextImpact call (modelSimulator.cpp:154-167): the
     // stall bound, S_TTS when a scattering time is due (drawn once, then drift_setup), else
@@ -337,8 +364,10 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
     };
 
     for (;;) {
-        // Pick the stage with the most waiting lanes (wave-uniform).  A scatter is always
-        // followed by a scattering-time draw, so the two share one stage.
+        // Pick the stage (wave-uniform): refill as soon as a lane has finished its phonon, since
+        // an empty lane idles through every other stage; otherwise the stage with the most
+        // waiting lanes.  A scatter is always followed by a scattering-time draw, so the two
+        // share one stage.
         const uint64_t m_fetch = __ballot(state == S_FETCH);
         const uint64_t m_ni = __ballot(state == S_NI);
         const uint64_t m_coll = __ballot(state == S_COLL);
@@ -350,38 +379,56 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
         if (n_ni > best) { best = n_ni; sel = S_NI; }
         if (n_scat > best) { best = n_scat; sel = S_SCAT; }
         if (n_coll > best) { best = n_coll; sel = S_COLL; }
-        if (n_fetch > best) { best = n_fetch; sel = S_FETCH; }
+        if (n_fetch > 0) { best = n_fetch; sel = S_FETCH; }
         if (sel == S_EXIT) { break; }  // every lane has exited
 
-        if (sel == S_FETCH) {
-            if (state == S_FETCH) {  // next phonon of the queue: one atomic per wave
+        if (sel == S_FETCH) {  // next phonons of the work queue
+            const uint32_t avail = res_end - res_next;
+            const uint32_t rank = (uint32_t)__popcll(m_fetch & lane_below);
+            int64_t item;
+            if ((uint32_t)n_fetch <= avail) {
+                item = (int64_t)res_next + rank;
+                res_next += (uint32_t)n_fetch;
+            } else {
+                const uint32_t need = (uint32_t)n_fetch - avail;
+                const uint32_t take = need > QUEUE_CLAIM ? need : QUEUE_CLAIM;
                 const int leader = __ffsll((unsigned long long)m_fetch) - 1;
                 uint32_t base = 0;
-                if (lane == leader) { base = atomicAdd(queue, (uint32_t)n_fetch); }
+                if (lane == leader) { base = atomicAdd(queue, take); }
                 base = __shfl(base, leader, WAVE);
-                const int64_t item = (int64_t)base + __popcll(m_fetch & lane_below);
+                item = rank < avail ? (int64_t)res_next + rank : (int64_t)base + (rank - avail);
+                res_next = base + need;
+                res_end = base + take;
+            }
+            if (state == S_FETCH) {
                 if (item >= n_items) {
                     state = S_EXIT;
                 } else {
+                    // started phonons from the dense buffer, phonons born in the window from the pool
                     const bool started = item < n_live;
-                    slot = started ? lists.live_in[item] : lists.born_order[b0 + (item - n_live)];
-                    const uint32_t meta = pool.meta[slot];
-                    p.cell = pool.cell[slot];
-                    p.px = pool.px[slot];
-                    p.py = pool.py[slot];
-                    p.dx = pool.dx[slot];
-                    p.dy = pool.dy[slot];
-                    p.freq = pool.freq[slot];
-                    p.vel = pool.vel[slot];
+                    const Live& L = lists.in;
+                    slot = started ? L.id[item] : lists.born_order[b0 + (item - n_live)];
+                    const int64_t j = started ? item : (int64_t)slot;
+                    const uint32_t meta = (started ? L.meta : pool.meta)[j];
+                    p.cell = (started ? L.cell : pool.cell)[j];
+                    cur_sensor = (started ? L.sensor : pool.sensor)[j];
+                    p.px = (started ? L.px : pool.px)[j];
+                    p.py = (started ? L.py : pool.py)[j];
+                    p.dx = (started ? L.dx : pool.dx)[j];
+                    p.dy = (started ? L.dy : pool.dy)[j];
+                    p.freq = (started ? L.freq : pool.freq)[j];
+                    p.vel = (started ? L.vel : pool.vel)[j];
+                    age = (started ? L.age : pool.age)[j];
+                    tts = (started ? L.tts : pool.tts)[j];
+                    if (started) {  // rates of the last scattering-time draw (a new phonon has none yet)
+                        relax[0] = L.rate0[item];
+                        relax[1] = L.rate1[item];
+                        relax[2] = L.rate2[item];
+                    }
                     p.bin = (int)(meta & META_BIN_MASK);
                     p.pol = (meta & META_TA) ? POL_TA : POL_LA;
                     p.sign = (meta & META_NEG) ? -1 : 1;
-                    age = pool.age[slot];
                     p.step = started ? k0 : (int32_t)(age / m.step_time);
-                    tts = pool.tts[slot];
-                    rate_sensor = (int)(meta >> META_RATE_SHIFT);
-                    cur_sensor = m.cells[p.cell].sensor;
-                    if (tts > 0.) { relax_rates(m, rate_sensor, p, relax); }
                     ttm = 0.;
                     events = 0;
                     psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, slot, (uint32_t)p.step);
@@ -436,15 +483,14 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
                 }
             }
             if (state == S_TTS) {
-                rate_sensor = cur_sensor;
-                relax_rates(m, rate_sensor, p, relax);
+                relax_rates(m, cur_sensor, p, relax);
                 tts = SCALING_FACTOR * -psim_log(psim_urand(&st)) / (0. + relax[0] + relax[1] + relax[2]);
                 drift_setup();
             }
         }
 
-        // Phonons that paused or died this iteration: survivors are appended to the next
-        // window's list (one atomic per wave), the dead leave the pool.
+        // Phonons that paused this iteration are appended to the next window's buffer (one
+        // atomic per wave); the dead leave.
         const bool survive = state == S_PAUSED;
         const uint64_t m_surv = __ballot(survive);
         if (m_surv) {
@@ -452,24 +498,30 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
             uint32_t base = 0;
             if (lane == leader) { base = atomicAdd(n_out, (uint32_t)__popcll(m_surv)); }
             base = __shfl(base, leader, WAVE);
-            if (survive) { lists.live_out[base + __popcll(m_surv & lane_below)] = slot; }
+            if (survive) {
+                const Live& O = lists.out;
+                const uint32_t k = base + __popcll(m_surv & lane_below);
+                O.px[k] = p.px;
+                O.py[k] = p.py;
+                O.dx[k] = p.dx;
+                O.dy[k] = p.dy;
+                O.freq[k] = p.freq;
+                O.vel[k] = p.vel;
+                O.age[k] = age;
+                O.tts[k] = tts;
+                O.rate0[k] = relax[0];
+                O.rate1[k] = relax[1];
+                O.rate2[k] = relax[2];
+                O.cell[k] = p.cell;
+                O.sensor[k] = cur_sensor;
+                O.meta[k] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
+                            | META_STARTED;
+                O.id[k] = slot;
+                state = S_FETCH;
+            }
         }
-        if (survive) {
-            pool.px[slot] = p.px;
-            pool.py[slot] = p.py;
-            pool.dx[slot] = p.dx;
-            pool.dy[slot] = p.dy;
-            pool.freq[slot] = p.freq;
-            pool.vel[slot] = p.vel;
-            pool.age[slot] = age;
-            pool.tts[slot] = tts;
-            pool.cell[slot] = p.cell;
-            pool.meta[slot] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
-                              | META_STARTED | ((uint32_t)rate_sensor << META_RATE_SHIFT);
-            state = S_FETCH;
-        } else if (state >= S_DEAD) {
+        if (state >= S_DEAD) {
             const int cause = (state - S_DEAD) / 8;
-            pool.cell[slot] = -1;
             fate_write(fate, slot, cause == FATE_EXPIRED ? p.step - 1 : p.step, cause, p.px, p.py);
             state = S_FETCH;
         }
@@ -557,7 +609,7 @@ struct psim_gpu_ctx {
     // birth ordering and live lists
     int32_t* d_birth_key = nullptr;
     uint32_t* d_born_order = nullptr;
-    uint32_t* d_live[2] = { nullptr, nullptr };
+    Live live[2] = {};  // dense state of the phonons alive between launch windows (ping-pong)
     int64_t list_cap = 0;
     uint32_t* d_birth_hist = nullptr;
     uint32_t* d_born_offset = nullptr;
@@ -594,7 +646,7 @@ void free_model(psim_gpu_ctx* c) {
 
 void free_pool(psim_gpu_ctx* c) {
     void* ps[] = { c->pool.px, c->pool.py, c->pool.dx, c->pool.dy, c->pool.freq, c->pool.vel, c->pool.age, c->pool.tts,
-        c->pool.cell, c->pool.meta };
+        c->pool.cell, c->pool.sensor, c->pool.meta };
     for (void* p : ps) {
         if (p) { (void)hipFree(p); }
     }
@@ -611,13 +663,20 @@ void free_fate(psim_gpu_ctx* c) {
 }
 
 void free_lists(psim_gpu_ctx* c) {
-    void* ps[] = { c->d_birth_key, c->d_born_order, c->d_live[0], c->d_live[1] };
+    void* ps[] = { c->d_birth_key, c->d_born_order };
     for (void* p : ps) {
         if (p) { (void)hipFree(p); }
     }
+    for (Live& L : c->live) {
+        void* ls[] = { L.px, L.py, L.dx, L.dy, L.freq, L.vel, L.age, L.tts, L.rate0, L.rate1, L.rate2, L.cell, L.sensor,
+            L.meta, L.id };
+        for (void* p : ls) {
+            if (p) { (void)hipFree(p); }
+        }
+        L = Live{};
+    }
     c->d_birth_key = nullptr;
     c->d_born_order = nullptr;
-    c->d_live[0] = c->d_live[1] = nullptr;
     c->list_cap = 0;
 }
 
@@ -706,9 +765,10 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
         c->pool.age = dalloc<double>(cap);
         c->pool.tts = dalloc<double>(cap);
         c->pool.cell = dalloc<int32_t>(cap);
+        c->pool.sensor = dalloc<int32_t>(cap);
         c->pool.meta = dalloc<uint32_t>(cap);
         if (!c->pool.px || !c->pool.py || !c->pool.dx || !c->pool.dy || !c->pool.freq || !c->pool.vel || !c->pool.age
-            || !c->pool.tts || !c->pool.cell || !c->pool.meta) {
+            || !c->pool.tts || !c->pool.cell || !c->pool.sensor || !c->pool.meta) {
             free_pool(c);
             return fail(c, PSIM_E_DEVICE, "hipMalloc failed for the phonon pool");
         }
@@ -731,9 +791,27 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
         const size_t cap = (size_t)std::max<int64_t>(n, 1);
         c->d_birth_key = dalloc<int32_t>(cap);
         c->d_born_order = dalloc<uint32_t>(cap);
-        c->d_live[0] = dalloc<uint32_t>(cap);
-        c->d_live[1] = dalloc<uint32_t>(cap);
-        if (!c->d_birth_key || !c->d_born_order || !c->d_live[0] || !c->d_live[1]) {
+        bool ok = c->d_birth_key && c->d_born_order;
+        for (Live& L : c->live) {
+            L.px = dalloc<double>(cap);
+            L.py = dalloc<double>(cap);
+            L.dx = dalloc<double>(cap);
+            L.dy = dalloc<double>(cap);
+            L.freq = dalloc<double>(cap);
+            L.vel = dalloc<double>(cap);
+            L.age = dalloc<double>(cap);
+            L.tts = dalloc<double>(cap);
+            L.rate0 = dalloc<double>(cap);
+            L.rate1 = dalloc<double>(cap);
+            L.rate2 = dalloc<double>(cap);
+            L.cell = dalloc<int32_t>(cap);
+            L.sensor = dalloc<int32_t>(cap);
+            L.meta = dalloc<uint32_t>(cap);
+            L.id = dalloc<uint32_t>(cap);
+            ok = ok && L.px && L.py && L.dx && L.dy && L.freq && L.vel && L.age && L.tts && L.rate0 && L.rate1 && L.rate2
+                 && L.cell && L.sensor && L.meta && L.id;
+        }
+        if (!ok) {
             free_lists(c);
             return fail(c, PSIM_E_DEVICE, "hipMalloc failed for the phonon lists");
         }
@@ -795,7 +873,7 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
     c->step_launches = 0;
     for (int32_t w = 0; w < (int32_t)n_windows && n > 0; ++w) {
         const int64_t k0 = windows[(size_t)w].first, k1 = windows[(size_t)w].second;
-        const Lists lists{ c->d_live[w % 2], c->d_live[(w + 1) % 2], c->d_n_live, c->d_queue, c->d_born_order,
+        const Lists lists{ c->live[w % 2], c->live[(w + 1) % 2], c->d_n_live, c->d_queue, c->d_born_order,
             c->d_born_offset, w };
         // window steps k0+1 .. k1 map to tally indices (k - adj) in [0, R)
         const bool rec = (k1 >= P.step_adjustment) && (k0 + 1 - P.step_adjustment < R) && (k1 - k0 <= W);

@@ -33,13 +33,11 @@ constexpr int GUIDE = 2048;  // guide-table cells per frequency table (a power o
 enum : int { POL_LA = 0, POL_TA = 1 };
 enum : int { FATE_ABSORBED = 1, FATE_EXPIRED = 2, FATE_NOT_STARTED = 3, FATE_STALLED = 4 };
 
-// Pool record word: bits 0-9 frequency bin, bit 10 TA, bit 11 negative sign,
-// bit 12 started, bits 16-31 sensor whose temperature the cached relaxation rates used.
+// Phonon record word: bits 0-9 frequency bin, bit 10 TA, bit 11 negative sign, bit 12 started.
 constexpr uint32_t META_BIN_MASK = 0x3ffu;
 constexpr uint32_t META_TA = 1u << 10;
 constexpr uint32_t META_NEG = 1u << 11;
 constexpr uint32_t META_STARTED = 1u << 12;
-constexpr int META_RATE_SHIFT = 16;
 
 struct Model {
     const psim_cell* __restrict__ cells;
@@ -78,6 +76,7 @@ struct Pool {
     double* __restrict__ age;
     double* __restrict__ tts;
     int32_t* __restrict__ cell;
+    int32_t* __restrict__ sensor;  // sensor of the cell
     uint32_t* __restrict__ meta;
 };
 
