@@ -39,6 +39,7 @@ constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
 constexpr int LDS_TALLY_BYTES = 48 * 1024;  // budget for one block's [W][S] window tally
 constexpr uint32_t QUEUE_CLAIM = 32;  // work-queue items a wave claims per atomic
+constexpr uint32_t OUT_CLAIM = 64;  // survivor slots a wave reserves per atomic
 
 struct Tally {
     int32_t* __restrict__ pcnt;  // [grid][W*S]
@@ -225,12 +226,15 @@ struct Live {
     uint32_t* __restrict__ id;
 };
 
-// Work of one transport launch: the started phonons (Live buffer `in`, n_live[window] entries)
-// followed by the phonons born in the window (born_order, read from the emission pool).
+// Work of one transport launch: the started phonons (Live buffer `in`, *n_in entries) followed by
+// the phonons born in the window (born_order, read from the emission pool).  Survivors go to
+// `out` in slots that waves reserve OUT_CLAIM at a time from *n_out; a wave marks the slots it
+// reserved but did not fill as holes (cell -1), which the cell sort drops.
 struct Lists {
     Live in;  // phonons alive at step k0 (started)
-    Live out;  // survivors at step k1
-    uint32_t* __restrict__ n_live;  // [windows + 1]: n_live[w] entries of `in`, n_live[w + 1] appended to `out`
+    Live out;  // survivors at step k1, with holes
+    const uint32_t* __restrict__ n_in;
+    uint32_t* __restrict__ n_out;
     uint32_t* __restrict__ queue;  // [windows]: work-queue cursor of launch w
     const uint32_t* __restrict__ born_order;  // ids sorted by birth step
     const uint32_t* __restrict__ born_offset;  // [N + 1]
@@ -272,10 +276,9 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
     LaneCounters c = { 0, 0, 0, 0, 0, 0, 0, 0 };
     const int lane = threadIdx.x % WAVE;
     const uint64_t lane_below = (1ull << lane) - 1ull;
-    const int64_t n_live = lists.n_live[lists.window];
+    const int64_t n_live = *lists.n_in;
     const int64_t b0 = lists.born_offset[k0];
     const int64_t n_items = n_live + (int64_t)lists.born_offset[k1] - b0;
-    uint32_t* __restrict__ n_out = lists.n_live + lists.window + 1;
     uint32_t* __restrict__ queue = lists.queue + lists.window;
 
     int state = S_FETCH;
@@ -296,6 +299,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
     // Work-queue items the wave has claimed but not handed out yet (wave-uniform): a refill takes
     // them in order and claims QUEUE_CLAIM more with one atomic when they run out.
     uint32_t res_next = 0, res_end = 0;
+    uint32_t out_next = 0, out_end = 0;  // survivor slots reserved by the wave, not filled yet
 
     // Top of simulatePhonon's loop up to the nextImpact call (modelSimulator.cpp:154-167): the
     // stall bound, S_TTS when a scattering time is due (drawn once, then drift_setup), else
@@ -489,18 +493,29 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
             }
         }
 
-        // Phonons that paused this iteration are appended to the next window's buffer (one
-        // atomic per wave); the dead leave.
+        // Phonons that paused this iteration go to the next window's buffer, in slots the wave
+        // has reserved; the dead leave.
         const bool survive = state == S_PAUSED;
         const uint64_t m_surv = __ballot(survive);
         if (m_surv) {
-            const int leader = __ffsll((unsigned long long)m_surv) - 1;
-            uint32_t base = 0;
-            if (lane == leader) { base = atomicAdd(n_out, (uint32_t)__popcll(m_surv)); }
-            base = __shfl(base, leader, WAVE);
+            const uint32_t n_surv = (uint32_t)__popcll(m_surv);
+            const uint32_t avail = out_end - out_next;
+            const uint32_t rank = (uint32_t)__popcll(m_surv & lane_below);
+            uint32_t k;
+            if (n_surv <= avail) {
+                k = out_next + rank;
+                out_next += n_surv;
+            } else {
+                const int leader = __ffsll((unsigned long long)m_surv) - 1;
+                uint32_t base = 0;
+                if (lane == leader) { base = atomicAdd(lists.n_out, OUT_CLAIM); }
+                base = __shfl(base, leader, WAVE);
+                k = rank < avail ? out_next + rank : base + (rank - avail);
+                out_next = base + (n_surv - avail);
+                out_end = base + OUT_CLAIM;
+            }
             if (survive) {
                 const Live& O = lists.out;
-                const uint32_t k = base + __popcll(m_surv & lane_below);
                 O.px[k] = p.px;
                 O.py[k] = p.py;
                 O.dx[k] = p.dx;
@@ -527,6 +542,9 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
         }
     }
 
+    // Reserved survivor slots the wave did not fill become holes.
+    for (uint32_t k = out_next + (uint32_t)lane; k < out_end; k += WAVE) { lists.out.cell[k] = -1; }
+
     // Counters: wave sums, then one global atomic per wave and counter.
     const uint32_t v[8] = { c.steps, c.scatters, c.impacts, c.relocations, c.absorbed, c.expired, c.tallies,
         c.stalled };
@@ -544,6 +562,110 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
             tally.pfx[base + i] = lfx[i];
             tally.pfy[base + i] = lfy[i];
         }
+    }
+}
+
+// ---- sort of the live buffer by cell ---------------------------------------------------------
+// The phonons a wave works on should sit in few cells, so that the edge records and tables they
+// read stay in L1: phonon ids are emitted cell by cell, but a window appends its survivors in
+// the order they pause.  Between windows a counting sort moves them back into cell order (key =
+// cell >> shift, at most SORT_KEYS keys): per-block LDS histograms, a scan, then a scatter in which
+// every block reserves its share of each key with one atomic.  Order inside a key is arbitrary;
+// no result depends on the order in which phonons are processed.
+constexpr int SORT_KEYS = 4096;
+constexpr int SORT_BLOCK = 1024;
+
+__device__ __forceinline__ void sort_range(uint32_t n, uint32_t& lo, uint32_t& hi) {
+    const uint64_t per = ((uint64_t)n + gridDim.x - 1) / gridDim.x;
+    lo = (uint32_t)std::min<uint64_t>(n, (uint64_t)blockIdx.x * per);
+    hi = (uint32_t)std::min<uint64_t>(n, (uint64_t)lo + per);
+}
+
+__global__ void __launch_bounds__(SORT_BLOCK) cell_hist_kernel(const int32_t* __restrict__ cell,
+    const uint32_t* __restrict__ n_ptr, int shift, int n_keys, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[SORT_KEYS];
+    for (int k = threadIdx.x; k < n_keys; k += SORT_BLOCK) { h[k] = 0; }
+    __syncthreads();
+    uint32_t lo, hi;
+    sort_range(*n_ptr, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_BLOCK) {
+        const int32_t cl = cell[i];
+        if (cl >= 0) { atomicAdd(&h[(uint32_t)cl >> shift], 1u); }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < n_keys; k += SORT_BLOCK) {
+        if (h[k]) { atomicAdd(&hist[k], h[k]); }
+    }
+}
+
+// Exclusive scan of the key counts in place (one block, n_keys <= SORT_KEYS); *total = their sum.
+__global__ void __launch_bounds__(1024) cell_scan_kernel(uint32_t* __restrict__ hist, int n_keys,
+    uint32_t* __restrict__ total) {
+    __shared__ uint32_t part[1024];
+    constexpr int PER = SORT_KEYS / 1024;
+    const int t = threadIdx.x;
+    uint32_t v[PER];
+    uint32_t sum = 0;
+    for (int q = 0; q < PER; ++q) {
+        const int k = t * PER + q;
+        v[q] = (k < n_keys) ? hist[k] : 0u;
+        sum += v[q];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t x = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (int q = 0; q < PER; ++q) {
+        const int k = t * PER + q;
+        if (k < n_keys) { hist[k] = run; }
+        run += v[q];
+    }
+    if (t == 1023) { *total = part[1023]; }
+}
+
+__global__ void __launch_bounds__(SORT_BLOCK) cell_scatter_kernel(Live src, Live dst,
+    const uint32_t* __restrict__ n_ptr, int shift, int n_keys, uint32_t* __restrict__ cursor) {
+    __shared__ uint32_t cnt[SORT_KEYS];
+    __shared__ uint32_t base[SORT_KEYS];
+    for (int k = threadIdx.x; k < n_keys; k += SORT_BLOCK) { cnt[k] = 0; }
+    __syncthreads();
+    uint32_t lo, hi;
+    sort_range(*n_ptr, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_BLOCK) {
+        const int32_t cl = src.cell[i];
+        if (cl >= 0) { atomicAdd(&cnt[(uint32_t)cl >> shift], 1u); }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < n_keys; k += SORT_BLOCK) {
+        base[k] = cnt[k] ? atomicAdd(&cursor[k], cnt[k]) : 0u;
+        cnt[k] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_BLOCK) {
+        const int32_t cl = src.cell[i];
+        if (cl < 0) { continue; }  // hole
+        const uint32_t key = (uint32_t)cl >> shift;
+        const uint32_t k = base[key] + atomicAdd(&cnt[key], 1u);
+        dst.px[k] = src.px[i];
+        dst.py[k] = src.py[i];
+        dst.dx[k] = src.dx[i];
+        dst.dy[k] = src.dy[i];
+        dst.freq[k] = src.freq[i];
+        dst.vel[k] = src.vel[i];
+        dst.age[k] = src.age[i];
+        dst.tts[k] = src.tts[i];
+        dst.rate0[k] = src.rate0[i];
+        dst.rate1[k] = src.rate1[i];
+        dst.rate2[k] = src.rate2[i];
+        dst.cell[k] = cl;
+        dst.sensor[k] = src.sensor[i];
+        dst.meta[k] = src.meta[i];
+        dst.id[k] = src.id[i];
     }
 }
 
@@ -593,6 +715,7 @@ struct psim_gpu_ctx {
     Model m{};
     std::vector<void*> model_allocs;
     int32_t n_sensors = 0;
+    int32_t n_cells = 0;
     // run buffers
     Pool pool{};
     int64_t pool_cap = 0;
@@ -616,6 +739,7 @@ struct psim_gpu_ctx {
     uint32_t* d_cursor = nullptr;
     uint32_t* d_n_live = nullptr;
     uint32_t* d_queue = nullptr;
+    uint32_t* d_sort_keys = nullptr;  // [SORT_KEYS] counts, then cursors, of the cell sort
     int64_t steps_cap = 0;
     int32_t window = 0;
     // timing of the last run
@@ -681,11 +805,11 @@ void free_lists(psim_gpu_ctx* c) {
 }
 
 void free_steps(psim_gpu_ctx* c) {
-    void* ps[] = { c->d_birth_hist, c->d_born_offset, c->d_cursor, c->d_n_live, c->d_queue };
+    void* ps[] = { c->d_birth_hist, c->d_born_offset, c->d_cursor, c->d_n_live, c->d_queue, c->d_sort_keys };
     for (void* p : ps) {
         if (p) { (void)hipFree(p); }
     }
-    c->d_birth_hist = c->d_born_offset = c->d_cursor = c->d_n_live = c->d_queue = nullptr;
+    c->d_birth_hist = c->d_born_offset = c->d_cursor = c->d_n_live = c->d_queue = c->d_sort_keys = nullptr;
     c->steps_cap = 0;
 }
 
@@ -786,9 +910,13 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
         }
         c->fate_cap = (int64_t)cap;
     }
-    if (c->list_cap < n) {
+    const int64_t blocks_needed = (n + BLOCK - 1) / BLOCK;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks_needed, (int64_t)c->n_cu * 8));
+    // A live buffer holds every phonon plus the survivor slots the waves may reserve but not fill.
+    const int64_t live_need = n + (int64_t)OUT_CLAIM * grid * (BLOCK / WAVE);
+    if (c->list_cap < live_need) {
         free_lists(c);
-        const size_t cap = (size_t)std::max<int64_t>(n, 1);
+        const size_t cap = (size_t)std::max<int64_t>(live_need, 1);
         c->d_birth_key = dalloc<int32_t>(cap);
         c->d_born_order = dalloc<uint32_t>(cap);
         bool ok = c->d_birth_key && c->d_born_order;
@@ -830,14 +958,13 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
         c->d_cursor = dalloc<uint32_t>((size_t)steps_need);
         c->d_n_live = dalloc<uint32_t>((size_t)steps_need);
         c->d_queue = dalloc<uint32_t>((size_t)steps_need);
-        if (!c->d_birth_hist || !c->d_born_offset || !c->d_cursor || !c->d_n_live || !c->d_queue) {
+        c->d_sort_keys = dalloc<uint32_t>((size_t)SORT_KEYS + 1);  // + the sorted count
+        if (!c->d_birth_hist || !c->d_born_offset || !c->d_cursor || !c->d_n_live || !c->d_queue || !c->d_sort_keys) {
             free_steps(c);
             return fail(c, PSIM_E_DEVICE, "hipMalloc failed for the step arrays");
         }
         c->steps_cap = steps_need;
     }
-    const int64_t blocks_needed = (n + BLOCK - 1) / BLOCK;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks_needed, (int64_t)c->n_cu * 8));
     const int64_t WS = (int64_t)W * S;
     if (c->partial_cap < grid * WS) {
         free_partials(c);
@@ -873,7 +1000,9 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
     c->step_launches = 0;
     for (int32_t w = 0; w < (int32_t)n_windows && n > 0; ++w) {
         const int64_t k0 = windows[(size_t)w].first, k1 = windows[(size_t)w].second;
-        const Lists lists{ c->live[w % 2], c->live[(w + 1) % 2], c->d_n_live, c->d_queue, c->d_born_order,
+        // window 0 starts from nothing (d_n_live[0] = 0); later ones from the sorted survivors
+        const uint32_t* n_in = (w == 0) ? c->d_n_live : c->d_sort_keys + SORT_KEYS;
+        const Lists lists{ c->live[0], c->live[1], n_in, c->d_n_live + w + 1, c->d_queue, c->d_born_order,
             c->d_born_offset, w };
         // window steps k0+1 .. k1 map to tally indices (k - adj) in [0, R)
         const bool rec = (k1 >= P.step_adjustment) && (k0 + 1 - P.step_adjustment < R) && (k1 - k0 <= W);
@@ -888,6 +1017,23 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
             const int rblocks = (int)((WS + BLOCK - 1) / BLOCK);
             hipLaunchKernelGGL(reduce_kernel, dim3(rblocks), dim3(BLOCK), 0, c->stream, c->d_pcnt, c->d_pfx, c->d_pfy,
                 grid, (int)WS, S, k0, P.step_adjustment, R, c->d_counts, c->d_flux);
+            HIP_CHECK(c, hipGetLastError());
+        }
+        if (w + 1 < (int32_t)n_windows) {  // survivors back into live[0], in cell order
+            int shift = 0;
+            while (((c->n_cells - 1) >> shift) >= SORT_KEYS) { ++shift; }
+            const int n_keys = ((c->n_cells - 1) >> shift) + 1;
+            const uint32_t* n_ptr = c->d_n_live + w + 1;
+            const int sgrid = std::max(1, c->n_cu * 2);
+            HIP_CHECK(c, hipMemsetAsync(c->d_sort_keys, 0, sizeof(uint32_t) * (size_t)n_keys, c->stream));
+            hipLaunchKernelGGL(cell_hist_kernel, dim3(sgrid), dim3(SORT_BLOCK), 0, c->stream, c->live[1].cell, n_ptr,
+                shift, n_keys, c->d_sort_keys);
+            HIP_CHECK(c, hipGetLastError());
+            hipLaunchKernelGGL(cell_scan_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_sort_keys, n_keys,
+                c->d_sort_keys + SORT_KEYS);
+            HIP_CHECK(c, hipGetLastError());
+            hipLaunchKernelGGL(cell_scatter_kernel, dim3(sgrid), dim3(SORT_BLOCK), 0, c->stream, c->live[1], c->live[0],
+                n_ptr, shift, n_keys, c->d_sort_keys);
             HIP_CHECK(c, hipGetLastError());
         }
         HIP_CHECK(c, hipEventSynchronize(c->ev[2]));
@@ -1099,6 +1245,7 @@ int psim_gpu_upload(psim_gpu_ctx* c, const psim_geom* g, const psim_tables* t, c
     c->d_flux = dalloc<double>((size_t)std::max<int64_t>(1, 2 * SR));
     if (!c->d_counts || !c->d_flux) { return fail(c, PSIM_E_DEVICE, "hipMalloc failed for tallies"); }
     c->n_sensors = g->n_sensors;
+    c->n_cells = g->n_cells;
     c->uploaded = true;
     return PSIM_OK;
 }
