@@ -6,6 +6,8 @@ own sources plus oracle/ref_driver.cpp) and runs it on three workloads:
   linear_ss     the 1000x100 nm bar, deviational steady state, 1e5 phonons, 2000 steps
   lshape_r      the conforming L, full mode, reduced to 1e5 phonons (5000 steps)
   film_r        the transient film, reduced to 1e5 phonons and 2000 steps over 2 ns
+  linear_k      the linear_ss bar at 1e6 phonons over 48 runs (statistics only, with every
+                run's fluxes): pins the effective conductivity to a few tenths of a percent
 
 For each workload it stores
   ref_<name>.json  the model, and over N independent reference runs (the
@@ -49,7 +51,11 @@ def workloads():
         "lshape_r": (configs.lshape_full(num_phonons=100_000), [300.0, 320.0, 280.0]),
         "film_r": (configs.film_transient(num_phonons=100_000, num_measurements=2000, sim_time=2.0),
                    [300.0, 320.0]),
+        "linear_k": (configs.linear_ss(num_phonons=1_000_000), None),
     }
+
+
+RUNS = {"linear_k": 48}  # runs per workload where the default is not enough
 
 
 def run_ref(model: dict, workdir: str, tag: str, extra=()) -> dict:
@@ -109,17 +115,20 @@ def main() -> None:
     for name, (model, temps) in workloads().items():
         if args.only and name != args.only:
             continue
+        n_runs = RUNS.get(name, args.runs)
         with tempfile.TemporaryDirectory() as wd:
-            state = run_ref(model, wd, "state", ["--tables", ",".join(str(t) for t in temps), "--geometry"])
+            state = None
+            if temps is not None:
+                state = run_ref(model, wd, "state", ["--tables", ",".join(str(t) for t in temps), "--geometry"])
             with ThreadPoolExecutor(args.jobs) as ex:
-                runs = list(ex.map(lambda i: run_ref(model, wd, f"run{i}"), range(args.runs)))
+                runs = list(ex.map(lambda i: run_ref(model, wd, f"run{i}"), range(n_runs)))
         s = model["settings"]
         keys = ["t_steady", "x_flux", "y_flux"]
         per = {k: np.array([[m[k] for m in r["measurements"]] for r in runs]) for k in keys}
         fix = {
             "name": name,
             "model": model,
-            "runs": args.runs,
+            "runs": n_runs,
             "eff_energy0": [r["eff_energy0"] for r in runs],
             "eff_energy": [r["eff_energy"] for r in runs],
             "total_phonons": [r["total_phonons"] for r in runs],
@@ -133,8 +142,13 @@ def main() -> None:
             fix["block_interval"] = interval
             fix["block_mean"] = series.mean(axis=0).tolist()
             fix["block_sd"] = series.std(axis=0, ddof=1).tolist()
+        if state is None:  # statistics only: keep every run's values
+            fix["per_run"] = {k: per[k].tolist() for k in keys}
         with open(os.path.join(HERE, f"ref_{name}.json"), "w") as f:
             json.dump(fix, f)
+        if state is None:
+            print(name, "mean wall", np.mean(fix["wall_s"]), "s")
+            continue
         arrays = {
             "temps": np.array(state["temps"]),
             "base_sums": np.array(state["base_sums"]),
