@@ -15,6 +15,12 @@ specularity 0.3, T_init 300 K), full (non-deviational) mode, 1e7 phonons, FP64 t
   cpu_baseline  the reference psim (oracle/_ref/psim_ref, built from the reference sources) on
              a bounded sample of the same workload (1e5 phonons), one single-threaded process on
              this host; its phonon-steps are counted by the CPU oracle on the same model.
+  time_to_steady_state  on the 1000x100 nm bar (10 sensors, walls 310/290 K, deviational, every
+             step recorded, 1e7 phonons, 2000 steps of 5 ps): the step after which the RMS over
+             sensors of the change in 100-step window-mean temperature stays below 0.05 K for 3
+             windows (psim_amd.analysis), then the same simulation cut at that step (phonon budget
+             scaled so the energy per phonon is unchanged) timed on the GPU, and its phonon-steps
+             at the reference build's rate on this host (sampled on the same bar at 1e5 phonons).
 
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
@@ -35,18 +41,16 @@ BYTES_PER_PHONON_STEP = 120.0
 HBM_PEAK_GBPS = 8000.0
 
 
-def cpu_baseline(sample_phonons: int) -> dict | None:
-    """Time the reference build (or, without it, the oracle) on a sample of the workload."""
+def reference_rate(cfg: dict) -> tuple:
+    """(phonon-steps/s, wall s, phonon-steps, kind) of one single-threaded reference run of cfg (the
+    oracle, single-threaded, where the reference build is absent); phonon-steps counted by the
+    oracle on the same model."""
     import oracle
     from oracle import refmodel
-    from psim_amd import configs
 
-    cfg = configs.lshape_full(num_phonons=sample_phonons)
     threads = max(1, min(16, os.cpu_count() or 1))
     counted = oracle.Model(cfg).run(0, nthreads=threads).counters["phonon_steps"]
     ref_bin = os.path.join(ROOT, "oracle", "_ref", "psim_ref")
-    sample = (f"L-shape full mode, {sample_phonons} phonons x 5000 steps of 2 ps, one process; "
-              f"{counted} phonon-steps counted by the oracle on the same model")
     if os.path.exists(ref_bin):
         with tempfile.TemporaryDirectory() as wd:
             txt = os.path.join(wd, "model.txt")
@@ -56,13 +60,52 @@ def cpu_baseline(sample_phonons: int) -> dict | None:
             subprocess.run([ref_bin, txt, out], check=True, stdout=subprocess.DEVNULL, timeout=600)
             with open(out) as f:
                 wall = json.load(f)["wall_s"]
-        return {"value": counted / wall, "unit": "phonon-steps/s", "cores": 1, "kind": "reference",
-                "sample": sample, "wall_s": wall}
+        return counted / wall, wall, counted, "reference"
     t0 = time.perf_counter()
     oracle.Model(cfg).run(0, nthreads=1)
     wall = time.perf_counter() - t0
-    return {"value": counted / wall, "unit": "phonon-steps/s", "cores": 1, "kind": "port", "sample": sample,
-            "wall_s": wall}
+    return counted / wall, wall, counted, "port"
+
+
+def cpu_baseline(sample_phonons: int) -> dict:
+    """The reference on a sample of the benchmark workload."""
+    from psim_amd import configs
+
+    rate, wall, counted, kind = reference_rate(configs.lshape_full(num_phonons=sample_phonons))
+    sample = (f"L-shape full mode, {sample_phonons} phonons x 5000 steps of 2 ps, one process; "
+              f"{counted} phonon-steps counted by the oracle on the same model")
+    return {"value": rate, "unit": "phonon-steps/s", "cores": 1, "kind": kind, "sample": sample, "wall_s": wall}
+
+
+def steady_state(engine, phonons: int, cpu: bool) -> dict:
+    """Time to steady state of the deviational bar on the GPU, and of the reference at its rate."""
+    import psim_amd
+    from psim_amd import analysis, configs
+
+    n_steps, sim_ns = 2000, 10.0
+    full = psim_amd.Model(configs.linear_bar(num_phonons=phonons, sim_type=1, step_interval=100).to_dict())
+    full.run(engine, 0)
+    steps = analysis.time_to_steady_state(full.results(0).temps)
+    out = {"workload": f"1000x100 nm Si bar, 10 sensors, walls 310/290 K, deviational (t_eq 300 K), every step "
+                       f"recorded, {phonons} phonons, {n_steps} steps of 5 ps",
+           "criterion": "RMS over sensors of the change in 100-step window-mean T < 0.05 K for 3 windows",
+           "steps": steps}
+    if steps is None:
+        return out
+    frac = steps / n_steps
+    cut = configs.linear_bar(num_phonons=int(round(phonons * frac)), num_measurements=steps, sim_time=sim_ns * frac,
+                             sim_type=1, step_interval=100).to_dict()
+    m = psim_amd.Model(cut)
+    m.run(engine, 1)
+    m.run(engine, 0)
+    gpu_s = engine.last_timing()["run_ms"] / 1e3
+    ps = m.results(0).counters["phonon_steps"]
+    out.update({"sim_ns": sim_ns * frac, "gpu_wall_s": gpu_s, "phonon_steps": ps})
+    if cpu:
+        rate, _, _, kind = reference_rate(configs.linear_bar(num_phonons=100_000, sim_type=1, step_interval=100).to_dict())
+        out.update({"cpu_wall_s": ps / rate, "cpu_rate": rate, "cpu_kind": kind, "cpu_cores": 1,
+                    "cpu_sample": "the same bar at 1e5 phonons, one process", "speedup": ps / rate / gpu_s})
+    return out
 
 
 def main() -> int:
@@ -73,6 +116,7 @@ def main() -> int:
     ap.add_argument("--phonons", type=int, default=10_000_000)
     ap.add_argument("--cpu-sample-phonons", type=int, default=100_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-steady-state", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -150,6 +194,8 @@ def main() -> int:
         line["roofline"]["traffic_bytes_per_phonon_step"] = tr["bytes_per_phonon_step"]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_phonons)
+    if not args.no_steady_state:
+        line["time_to_steady_state"] = steady_state(engine, 10_000_000, not args.no_cpu_baseline)
     print(json.dumps(line), flush=True)
     engine.close()
     return 0
