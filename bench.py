@@ -22,7 +22,12 @@ specularity 0.3, T_init 300 K), full (non-deviational) mode, 1e7 phonons, FP64 t
              scaled so the energy per phonon is unchanged) timed on the GPU, and its phonon-steps
              at the reference build's rate on this host (sampled on the same bar at 1e5 phonons).
 
-Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+With --workload film it measures the transient case instead: the 1500x150 nm film (30 sensors,
+baseline 300 K, a 20 K burst for 100 ps into sensors 0-2), 1e8 phonons, 20000 steps of 1 ps, every
+step tallied (snapshots every 100 steps), one run after no warmup; its CPU baseline is the
+reference on the same film at 1e4 phonons.
+
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload lshape|film]
 """
 from __future__ import annotations
 
@@ -67,13 +72,29 @@ def reference_rate(cfg: dict) -> tuple:
     return counted / wall, wall, counted, "port"
 
 
-def cpu_baseline(sample_phonons: int) -> dict:
-    """The reference on a sample of the benchmark workload."""
+def workload(name: str, phonons: int) -> tuple:
+    """(model dict, config description) of a benchmark workload."""
     from psim_amd import configs
 
-    rate, wall, counted, kind = reference_rate(configs.lshape_full(num_phonons=sample_phonons))
-    sample = (f"L-shape full mode, {sample_phonons} phonons x 5000 steps of 2 ps, one process; "
-              f"{counted} phonon-steps counted by the oracle on the same model")
+    if name == "film":
+        cfg = configs.film_transient(num_phonons=phonons)
+        desc = {"workload": "1500x150 nm Si film (30 sensors, 60 triangles), transient deviational, baseline "
+                            "300 K, 20 K burst for 100 ps into sensors 0-2, snapshots every 100 steps",
+                "phonons": phonons, "measurement_steps": 20000, "dt_ps": 1.0}
+    else:
+        cfg = configs.lshape_full(num_phonons=phonons)
+        desc = {"workload": "L-shaped Si (40 sensors, 80 triangles), full-mode steady state, walls 320/280 K, "
+                            "specularity 0.3", "phonons": phonons, "measurement_steps": 5000, "dt_ps": 2.0}
+    desc.update({"tallies": "fp64", "parallelism": "single GPU"})
+    return cfg, desc
+
+
+def cpu_baseline(name: str, sample_phonons: int) -> dict:
+    """The reference on a sample of the benchmark workload."""
+    cfg, desc = workload(name, sample_phonons)
+    rate, wall, counted, kind = reference_rate(cfg)
+    sample = (f"{desc['workload']}, {sample_phonons} phonons x {desc['measurement_steps']} steps of "
+              f"{desc['dt_ps']} ps, one process; {counted} phonon-steps counted by the oracle on the same model")
     return {"value": rate, "unit": "phonon-steps/s", "cores": 1, "kind": kind, "sample": sample, "wall_s": wall}
 
 
@@ -111,13 +132,25 @@ def steady_state(engine, phonons: int, cpu: bool) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--phonons", type=int, default=10_000_000)
-    ap.add_argument("--cpu-sample-phonons", type=int, default=100_000)
+    ap.add_argument("--workload", choices=["lshape", "film"], default="lshape")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--phonons", type=int, default=None)
+    ap.add_argument("--cpu-sample-phonons", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-steady-state", action="store_true")
     args = ap.parse_args()
+    film = args.workload == "film"
+    if args.steps is None:
+        args.steps = 1 if film else 3
+    if args.warmup is None:
+        args.warmup = 0 if film else 1
+    if args.phonons is None:
+        args.phonons = 100_000_000 if film else 10_000_000
+    if args.cpu_sample_phonons is None:
+        args.cpu_sample_phonons = 10_000 if film else 100_000
+    if film:
+        os.environ.setdefault("PSIM_PROGRESS", "1")  # one stderr line per launch window
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (the multi-GPU design, sharding "
@@ -126,9 +159,8 @@ def main() -> int:
         return 2
 
     import psim_amd
-    from psim_amd import configs
 
-    cfg = configs.lshape_full(num_phonons=args.phonons)
+    cfg, desc = workload(args.workload, args.phonons)
     engine = psim_amd.Engine(0)
     model = psim_amd.Model(cfg)
     _, total = model.upload(engine, 0)
@@ -161,15 +193,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {
-            "workload": "L-shaped Si (40 sensors, 80 triangles), full-mode steady state, walls 320/280 K, "
-                        "specularity 0.3",
-            "phonons": total,
-            "measurement_steps": 5000,
-            "dt_ps": 2.0,
-            "tallies": "fp64",
-            "parallelism": "single GPU",
-        },
+        "config": dict(desc, phonons=total),
         "roofline": {
             "bound": "hbm",
             "kernel": "transport_kernel",
@@ -187,14 +211,14 @@ def main() -> int:
     # HBM traffic of the transport kernel per launch, measured by rocprofv3 --pmc FETCH_SIZE and
     # WRITE_SIZE (scaled by the copy-kernel calibration) on this workload; see profiles/.
     traffic_file = os.path.join(ROOT, "profiles", "transport_kernel_lshape_1e7_traffic.json")
-    if args.phonons == 10_000_000 and os.path.exists(traffic_file):
+    if not film and args.phonons == 10_000_000 and os.path.exists(traffic_file):
         with open(traffic_file) as f:
             tr = json.load(f)
         line["roofline"]["traffic"] = tr["bytes_per_dispatch"]
         line["roofline"]["traffic_bytes_per_phonon_step"] = tr["bytes_per_phonon_step"]
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_sample_phonons)
-    if not args.no_steady_state:
+        line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_sample_phonons)
+    if not args.no_steady_state and not film:
         line["time_to_steady_state"] = steady_state(engine, 10_000_000, not args.no_cpu_baseline)
     print(json.dumps(line), flush=True)
     engine.close()

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -998,6 +999,7 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
     }
     c->step_ms = 0.;
     c->step_launches = 0;
+    const bool progress = std::getenv("PSIM_PROGRESS") != nullptr;  // one stderr line per window
     for (int32_t w = 0; w < (int32_t)n_windows && n > 0; ++w) {
         const int64_t k0 = windows[(size_t)w].first, k1 = windows[(size_t)w].second;
         // window 0 starts from nothing (d_n_live[0] = 0); later ones from the sorted survivors
@@ -1041,6 +1043,11 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
         HIP_CHECK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
         c->step_ms += ms;
         c->step_launches++;
+        if (progress) {
+            std::fprintf(stderr, "psim_gpu: window %d/%d steps [%lld, %lld) %.1f ms\n", (int)w + 1, (int)n_windows,
+                (long long)k0, (long long)k1, (double)ms);
+            std::fflush(stderr);
+        }
     }
     HIP_CHECK(c, hipEventRecord(c->ev[3], c->stream));
     HIP_CHECK(c, hipEventSynchronize(c->ev[3]));

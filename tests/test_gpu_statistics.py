@@ -4,8 +4,11 @@ The reference's RNG cannot be seeded, so agreement is statistical: over 16 GPU s
 reference runs every sensor's mean temperature and x/y flux must agree within 3 standard errors
 of the difference, |mean_gpu - mean_ref| <= 3 sqrt(s_gpu^2/16 + s_ref^2/16).  The transient
 film is also compared block by block (100-step block means, the periodic export's series):
-there the 3-sigma bound is applied to each of the 600 entries as a family, allowing the
-expected few exceedances (at most 1 %) and none beyond 4.5 sigma.  The effective
+there the 3-sigma bound is applied to the entries as a family: with standard errors estimated
+from 16 runs each z follows Student's t (15 degrees of freedom per side), so the number of
+entries beyond 3 may not exceed the 99.9th percentile of Binomial(n, P(|t| > 3)) and the
+largest z may not exceed the bound a family of n entries passes with probability 99.9 %.  The same film with the
+benchmark's 1e8-phonon pool is compared with the reference means over the fixture's 2000 steps.  The effective
 conductivity of the 1000 nm bar (1e6 phonons, 48 runs on each side) must agree within 1 %.
 """
 import json
@@ -13,6 +16,7 @@ import os
 
 import numpy as np
 import pytest
+from scipy import stats
 
 import psim_amd
 from psim_amd import analysis
@@ -25,6 +29,16 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 def _fixture(name):
     with open(os.path.join(GOLDEN, f"ref_{name}.json")) as f:
         return json.load(f)
+
+
+def _family_ok(z, dof):
+    """(passes, detail) for a family of 3-sigma comparisons whose z values are Student-t(dof)."""
+    z = np.asarray(z).ravel()
+    n = z.size
+    allowed = stats.binom.ppf(0.999, n, 2 * stats.t.sf(3.0, dof))
+    z_max = stats.t.isf(0.0005 / n, dof)
+    over = int(np.sum(z > 3.0))
+    return over <= allowed and z.max() <= z_max, (n, over, allowed, round(float(z.max()), 2), round(float(z_max), 2))
 
 
 @pytest.fixture(scope="module")
@@ -56,7 +70,9 @@ def test_sensor_means_within_three_sigma(engine, name):
     if "block_mean" in fix:
         blocks = [analysis.block_means(r.temps, fix["block_interval"]) for r in runs]
         z = analysis.three_sigma(blocks, fix["block_mean"], fix["block_sd"], fix["runs"])
-        assert np.mean(z > 3.0) <= 0.01 and z.max() <= 4.5, (np.mean(z > 3.0), z.max())
+        ok, detail = _family_ok(z, 2 * (fix["runs"] - 1))
+        print("block family (n, over 3, allowed, max z, bound):", detail, flush=True)
+        assert ok, detail
 
 
 def test_effective_conductivity_within_one_percent(engine):
@@ -70,3 +86,36 @@ def test_effective_conductivity_within_one_percent(engine):
     print(f"k_eff gpu {k_gpu.mean():.5g} ref {k_ref.mean():.5g} rel diff {rel:.4%} (se {se:.4%})", flush=True)
     assert rel <= 0.01
     assert rel <= 3 * se
+
+
+def test_film_at_full_pool_matches_reference(engine):
+    """The transient film with the 1e8-phonon pool of the benchmark case, over the 2000 steps the
+    reference fixture covers: the phonon count is the only difference (the burst energy and so
+    the temperatures do not depend on it), so one GPU run must reproduce the reference's 16-run
+    means; its own spread is ~30x below one reference run's and is neglected."""
+    from psim_amd import configs
+
+    fix = _fixture("film_r")
+    cfg = configs.film_transient(num_phonons=100_000_000, num_measurements=2000, sim_time=2.0)
+    print("film_r at 1e8 phonons", flush=True)
+    m = psim_amd.Model(cfg)
+    m.run(engine, 0)
+    r = m.results(0)
+    se = lambda sd: np.asarray(sd) / np.sqrt(fix["runs"])  # noqa: E731
+    for col, key in [(0, "t_steady"), (2, "x_flux"), (4, "y_flux")]:
+        z = np.abs(r.measurements[:, col] - np.asarray(fix["mean"][key])) / se(fix["sd"][key])
+        assert np.all(z <= 3.0), (key, np.round(z, 2))
+    blocks = analysis.block_means(r.temps, fix["block_interval"])
+    delta = np.abs(blocks - np.asarray(fix["block_mean"]))
+    sd = np.asarray(fix["block_sd"])
+    # Where the burst has barely arrived (far sensors, early blocks) the 16 reference runs hold a
+    # handful of stray phonons, or none: their sample spread is below 0.01 K and no estimate of the
+    # standard error.  Those entries are compared in kelvin instead (0.01 K).
+    quiet = sd < 0.01
+    print(f"blocks with reference spread < 0.01 K: {int(quiet.sum())}, max |dT| there "
+          f"{delta[quiet].max(initial=0):.5f} K", flush=True)
+    assert np.all(delta[quiet] <= 0.01), delta[quiet]
+    z = delta[~quiet] / se(sd[~quiet])
+    ok, detail = _family_ok(z, fix["runs"] - 1)
+    print("block family (n, over 3, allowed, max z, bound):", detail, flush=True)
+    assert ok, detail
