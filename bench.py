@@ -27,7 +27,14 @@ baseline 300 K, a 20 K burst for 100 ps into sensors 0-2), 1e8 phonons, 20000 st
 step tallied (snapshots every 100 steps), one run after no warmup; its CPU baseline is the
 reference on the same film at 1e4 phonons.
 
-Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload lshape|film]
+With --workload batch it measures batched geometries: 256 independent structures from
+psim_amd.configs.batch_geometries(seed 0) (straight or kinked Si bars, 4-200 sensors, walls
+U(250, 350) K, specularity U(0, 1), 1e4-1e6 phonons each, deviational steady state, 2000 steps of
+5 ps) run as one engine upload and one run (psim_batch_run).  `value` is their phonon-steps per
+second of GPU run time; the same models run one after the other on the GPU, and the reference on
+a sample of them (scaled to 1e4 phonons each), are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload lshape|film|batch]
 """
 from __future__ import annotations
 
@@ -129,10 +136,76 @@ def steady_state(engine, phonons: int, cpu: bool) -> dict:
     return out
 
 
+def build_models(cfgs: list) -> list:
+    """psim_amd.Model of every config, built on host threads (ctypes releases the GIL)."""
+    import psim_amd
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        return list(ex.map(psim_amd.Model, cfgs))
+
+
+def batch_bench(args) -> dict:
+    import psim_amd
+    from psim_amd import configs
+
+    cfgs = configs.batch_geometries(seed=0, count=256)
+    engine = psim_amd.Engine(0)
+    t0 = time.perf_counter()
+    models = build_models(cfgs)
+    t_build = time.perf_counter() - t0
+    for w in range(args.warmup):
+        psim_amd.run_batch(engine, models, seed=10_000 + w)
+    gpu_s = wall = 0.0
+    steps_total = 0
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        counters = psim_amd.run_batch(engine, models, seed=k)
+        wall += time.perf_counter() - t0
+        gpu_s += engine.last_timing()["run_ms"] / 1e3
+        steps_total += counters["phonon_steps"]
+    value = steps_total / gpu_s
+    # the same models one at a time (each its own upload and run)
+    seq_steps, seq_gpu = 0, 0.0
+    for m in models:
+        m.run(engine, 0)
+        seq_steps += m.results(0).counters["phonon_steps"]
+        seq_gpu += engine.last_timing()["run_ms"] / 1e3
+    line = {
+        "metric": "phonon-steps/s", "value": value, "unit": "phonon-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": gpu_s * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "256 independent Si structures (psim_amd.configs.batch_geometries seed 0): straight or "
+                               "kinked bars, 4-200 sensors, walls U(250,350) K, specularity U(0,1), 1e4-1e6 "
+                               "phonons each, deviational steady state, 2000 steps of 5 ps, one batched run",
+                   "models": len(models), "phonons": int(sum(c["settings"]["num_phonons"] for c in cfgs)),
+                   "tallies": "fp64", "parallelism": "single GPU"},
+        "batch": {"gpu_run_s": gpu_s / args.steps, "end_to_end_s": wall / args.steps,
+                  "host_model_build_s": t_build, "phonon_steps": steps_total // args.steps},
+        "sequential_gpu": {"value": seq_steps / seq_gpu, "gpu_run_s": seq_gpu, "phonon_steps": seq_steps,
+                           "batch_speedup": value / (seq_steps / seq_gpu)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        import copy
+        rate_steps, rate_wall = 0, 0.0
+        for cfg in cfgs[:4]:
+            c = copy.deepcopy(cfg)
+            c["settings"]["num_phonons"] = 10_000
+            _, w, n, kind = reference_rate(c)
+            rate_steps += n
+            rate_wall += w
+        line["cpu_baseline"] = {"value": rate_steps / rate_wall, "unit": "phonon-steps/s", "cores": 1, "kind": kind,
+                                "sample": "the first 4 structures of the batch at 1e4 phonons each, one process, "
+                                          "run one after the other", "wall_s": rate_wall}
+    engine.close()
+    return line
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["lshape", "film"], default="lshape")
+    ap.add_argument("--workload", choices=["lshape", "film", "batch"], default="lshape")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--phonons", type=int, default=None)
@@ -157,6 +230,10 @@ def main() -> int:
               "phonon ids and all-reducing the tallies, is described in DESIGN.md and not run here).",
               file=sys.stderr)
         return 2
+
+    if args.workload == "batch":
+        print(json.dumps(batch_bench(args)), flush=True)
+        return 0
 
     import psim_amd
 

@@ -125,7 +125,10 @@ typedef struct {
 } psim_tables;
 
 /* An emission source: a cell-origin or surface-origin builder
- * (psim/include/psim/phononBuilder.h:37-72); phonon ids [offset, offset + count). */
+ * (psim/include/psim/phononBuilder.h:37-72); phonon ids [offset, offset + count).
+ * A phonon's random streams are keyed by (stream, id - id_base): a single model uses stream 0
+ * and id_base 0; a batch of models gives model m stream m and id_base = its first phonon id, so
+ * every model draws exactly the numbers of a single run with the same stream index. */
 typedef struct {
     int32_t kind; /* 0 cell origin, 1 surface origin */
     int32_t cell;
@@ -135,6 +138,9 @@ typedef struct {
     double start, duration; /* surface: birth window */
     double p1x, p1y, p2x, p2y; /* surface: emitting line */
     double nx, ny; /* surface: inward normal */
+    int32_t stream; /* 0 .. 65535 */
+    int32_t pad_;
+    int64_t id_base; /* <= offset */
 } psim_source;
 
 typedef struct {
@@ -247,6 +253,19 @@ int psim_model_budget(const psim_model* model, double* eff_energy, int64_t* tota
  * `json_path`, averaged over runs.  The periodic writer averages the per-step series over runs
  * (the reference's calculateAverages drops them and then faults, outputManager.cpp:40-93). */
 int psim_model_export(const psim_model* model, const char* json_path, double time_s);
+
+/* Random-stream index of the model's phonons (default 0).  Stream m reproduces model m of a
+ * batch (psim_batch_run) exactly. */
+int psim_model_set_stream(psim_model* model, int32_t stream);
+
+/* Batched geometries (no counterpart in the reference, which runs model files one after the
+ * other, psim/src/main.cpp:13-25): runs n independent models as one engine upload and one run.
+ * Every model keeps its own tables, budget (its own energy per phonon) and results, readable with
+ * psim_model_results(model, 0, ...); model m draws the random numbers of a single run with
+ * stream m.  The models must share num_measurements, sim_time, sim_type, step_interval and the
+ * phasor flag, and have num_runs 1.  `counters` (may be null) receives the batch totals. */
+int psim_batch_run(psim_model* const* models, int32_t n, psim_gpu_ctx* ctx, uint64_t seed, int64_t* counters,
+    char* err, int errlen);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,9 @@ def lib():
         "psim_model_budget": (ctypes.c_int, [_VP, _D, _I64]),
         "psim_model_prepare": (ctypes.c_int, [_VP, ctypes.c_uint64, _D, _I64, _I64, ctypes.c_int32, _I32]),
         "psim_model_export": (ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_double]),
+        "psim_model_set_stream": (ctypes.c_int, [_VP, ctypes.c_int32]),
+        "psim_batch_run": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_int32, _VP, ctypes.c_uint64,
+                                          ctypes.POINTER(ctypes.c_int64), ctypes.c_char_p, ctypes.c_int]),
         "psim_model_upload": (ctypes.c_int, [_VP, _VP, ctypes.c_uint64, _D, _I64]),
     }
     for name, (res, args) in sig.items():
@@ -236,3 +239,20 @@ class Model:
 
     def export(self, json_path: str, time_s: float) -> None:
         self._check(self._L.psim_model_export(self._h, json_path.encode(), float(time_s)))
+
+    def set_stream(self, stream: int) -> None:
+        """Random-stream index of this model's phonons; stream m reproduces model m of a batch."""
+        self._check(self._L.psim_model_set_stream(self._h, int(stream)))
+
+
+def run_batch(engine: Engine, models: list, seed: int) -> dict:
+    """Runs independent models as one engine upload and one run (psim_batch_run); each model's
+    results are then read with model.results(0).  Returns the batch's counters."""
+    L = lib()
+    handles = (_VP * len(models))(*[m._h for m in models])
+    counters = np.zeros(len(COUNTER_NAMES), np.int64)
+    err = ctypes.create_string_buffer(2048)
+    rc = L.psim_batch_run(handles, len(models), engine.handle, int(seed), _p(counters, _I64), err, 2048)
+    if rc != 0:
+        raise PsimError(err.value.decode())
+    return dict(zip(COUNTER_NAMES, counters.tolist()))

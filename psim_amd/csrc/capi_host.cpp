@@ -55,6 +55,35 @@ int psim_model_run(psim_model* m, psim_gpu_ctx* ctx, uint64_t seed) {
     } catch (const std::exception& e) { return model_fail(m, PSIM_E_DEVICE, e.what()); }
 }
 
+int psim_model_set_stream(psim_model* m, int32_t stream) {
+    if (!m || stream < 0 || stream > 0xffff) { return PSIM_E_INVALID; }
+    m->model->setStream(stream);
+    return PSIM_OK;
+}
+
+int psim_batch_run(psim_model* const* models, int32_t n, psim_gpu_ctx* ctx, uint64_t seed, int64_t* counters,
+    char* err, int errlen) {
+    auto fail_msg = [&](int code, const std::string& msg) {
+        if (err && errlen > 0) {
+            std::strncpy(err, msg.c_str(), (size_t)errlen - 1);
+            err[errlen - 1] = '\0';
+        }
+        return code;
+    };
+    if (!models || n <= 0 || !ctx) { return fail_msg(PSIM_E_INVALID, "batch needs models and an engine"); }
+    std::vector<psim::Model*> list;
+    for (int32_t i = 0; i < n; ++i) {
+        if (!models[i]) { return fail_msg(PSIM_E_INVALID, "null model in batch"); }
+        list.push_back(models[i]->model.get());
+    }
+    try {
+        std::array<int64_t, PSIM_NUM_COUNTERS> c{};
+        psim::Model::runBatch(list, ctx, seed, c);
+        if (counters) { std::memcpy(counters, c.data(), sizeof c); }
+        return PSIM_OK;
+    } catch (const std::exception& e) { return fail_msg(PSIM_E_DEVICE, e.what()); }
+}
+
 int psim_model_results(const psim_model* m, int64_t run, double* out, double* temps, double* fluxes, int64_t* counts,
     double* raw_flux, int64_t* counters) {
     if (!m) { return PSIM_E_INVALID; }

@@ -39,14 +39,22 @@ namespace {
 constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
 constexpr int LDS_TALLY_BYTES = 48 * 1024;  // budget for one block's [W][S] window tally
+constexpr int GLOBAL_TALLY_STEPS = 8;  // fewer LDS-tallied steps per window than this: global atomics
 constexpr uint32_t QUEUE_CLAIM = 32;  // work-queue items a wave claims per atomic
 constexpr uint32_t OUT_CLAIM = 64;  // survivor slots a wave reserves per atomic
 
+// Tallies of a launch window.  Normally privatised: a [W][S] tally per block in LDS, summed into
+// per-block partials.  When even GLOBAL_TALLY_STEPS steps of S sensors do not fit the LDS budget
+// (thousands of sensors, e.g. a batch of geometries), every tally goes straight to the [S][R]
+// arrays with atomics instead; with that many sensors they rarely collide.
 struct Tally {
     int32_t* __restrict__ pcnt;  // [grid][W*S]
     double* __restrict__ pfx;
     double* __restrict__ pfy;
+    int64_t* __restrict__ counts;  // [S][R] (global mode)
+    double* __restrict__ flux;  // [S][R][2] (global mode)
     int32_t rec;  // 1 if this window records
+    int32_t global;  // 1: atomics into counts / flux
     int32_t W;
 };
 
@@ -91,8 +99,9 @@ __global__ void __launch_bounds__(BLOCK) emit_kernel(Model m, Pool pool, int64_t
         const psim_source& src = m.sources[lo];
         const psim_cell& cl = m.cells[src.cell];
         const int mat = m.sensors[cl.sensor].material;
+        const uint32_t sid = (uint32_t)(id - src.id_base);
         psim_stream st;
-        psim_stream_init(&st, seed, PSIM_TAG_EMIT, (uint32_t)id, 0);
+        psim_stream_init(&st, seed, psim_stream_tag(PSIM_TAG_EMIT, (uint32_t)src.stream), sid, 0);
         Ph p;
         p.cell = src.cell;
         p.sign = src.sign;
@@ -145,7 +154,9 @@ __global__ void __launch_bounds__(BLOCK) emit_kernel(Model m, Pool pool, int64_t
         pool.tts[id] = 0.;
         pool.cell[id] = cell;
         pool.sensor[id] = cl.sensor;
-        pool.meta[id] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u);
+        pool.sid[id] = sid;
+        pool.meta[id] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
+                        | ((uint32_t)src.stream << META_STREAM_SHIFT);
     }
     // birth-step histogram; most waves of a cell-origin source share one key (step 0)
     const int lane = threadIdx.x % WAVE;
@@ -225,6 +236,7 @@ struct Live {
     int32_t* __restrict__ sensor;  // sensor of the cell
     uint32_t* __restrict__ meta;
     uint32_t* __restrict__ id;
+    uint32_t* __restrict__ sid;  // id within the phonon's random stream
 };
 
 // Work of one transport launch: the started phonons (Live buffer `in`, *n_in entries) followed by
@@ -263,7 +275,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
     unsigned long long* __restrict__ counters, Fate fate) {
     extern __shared__ unsigned char lds_raw[];
     const int S = m.n_sensors;
-    const int WS = tally.rec ? tally.W * S : 0;
+    const int WS = (tally.rec && !tally.global) ? tally.W * S : 0;
     double* lfx = reinterpret_cast<double*>(lds_raw);
     double* lfy = lfx + WS;
     int32_t* lcnt = reinterpret_cast<int32_t*>(lfy + WS);
@@ -284,6 +296,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
 
     int state = S_FETCH;
     uint32_t slot = 0;
+    uint32_t sid = 0, stream = 0;  // the phonon's random stream: (stream index, id within it)
     Ph p;
     p.px = p.py = p.dx = p.dy = p.freq = p.vel = 0.;
     p.cell = p.bin = p.pol = 0;
@@ -342,11 +355,19 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
                 if (p.step >= m.adj) {  // Sensor::updateHeatParams, sensor.cpp:43-52
                     const int idx = p.step - m.adj;
                     if (idx < m.R && tally.rec) {
-                        const int e = (p.step - k0 - 1) * S + cur_sensor;
                         const double sg = (double)p.sign;
-                        atomicAdd(&lcnt[e], p.sign);
-                        atomicAdd(&lfx[e], p.dx * p.vel * sg);
-                        atomicAdd(&lfy[e], p.dy * p.vel * sg);
+                        if (tally.global) {
+                            const int64_t t = (int64_t)cur_sensor * m.R + idx;
+                            atomicAdd(reinterpret_cast<unsigned long long*>(&tally.counts[t]),
+                                (unsigned long long)(int64_t)p.sign);
+                            atomicAdd(&tally.flux[2 * t], p.dx * p.vel * sg);
+                            atomicAdd(&tally.flux[2 * t + 1], p.dy * p.vel * sg);
+                        } else {
+                            const int e = (p.step - k0 - 1) * S + cur_sensor;
+                            atomicAdd(&lcnt[e], p.sign);
+                            atomicAdd(&lfx[e], p.dx * p.vel * sg);
+                            atomicAdd(&lfy[e], p.dy * p.vel * sg);
+                        }
                         c.tallies++;
                     }
                 }
@@ -415,6 +436,8 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
                     slot = started ? L.id[item] : lists.born_order[b0 + (item - n_live)];
                     const int64_t j = started ? item : (int64_t)slot;
                     const uint32_t meta = (started ? L.meta : pool.meta)[j];
+                    sid = (started ? L.sid : pool.sid)[j];
+                    stream = meta >> META_STREAM_SHIFT;
                     p.cell = (started ? L.cell : pool.cell)[j];
                     cur_sensor = (started ? L.sensor : pool.sensor)[j];
                     p.px = (started ? L.px : pool.px)[j];
@@ -436,7 +459,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
                     p.step = started ? k0 : (int32_t)(age / m.step_time);
                     ttm = 0.;
                     events = 0;
-                    psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, slot, (uint32_t)p.step);
+                    psim_stream_init(&st, seed, psim_stream_tag(PSIM_TAG_TRANSPORT, stream), sid, (uint32_t)p.step);
                     begin_event();
                 }
             }
@@ -531,8 +554,9 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
                 O.cell[k] = p.cell;
                 O.sensor[k] = cur_sensor;
                 O.meta[k] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
-                            | META_STARTED;
+                            | META_STARTED | (stream << META_STREAM_SHIFT);
                 O.id[k] = slot;
+                O.sid[k] = sid;
                 state = S_FETCH;
             }
         }
@@ -667,6 +691,7 @@ __global__ void __launch_bounds__(SORT_BLOCK) cell_scatter_kernel(Live src, Live
         dst.sensor[k] = src.sensor[i];
         dst.meta[k] = src.meta[i];
         dst.id[k] = src.id[i];
+        dst.sid[k] = src.sid[i];
     }
 }
 
@@ -771,7 +796,7 @@ void free_model(psim_gpu_ctx* c) {
 
 void free_pool(psim_gpu_ctx* c) {
     void* ps[] = { c->pool.px, c->pool.py, c->pool.dx, c->pool.dy, c->pool.freq, c->pool.vel, c->pool.age, c->pool.tts,
-        c->pool.cell, c->pool.sensor, c->pool.meta };
+        c->pool.cell, c->pool.sensor, c->pool.meta, c->pool.sid };
     for (void* p : ps) {
         if (p) { (void)hipFree(p); }
     }
@@ -794,7 +819,7 @@ void free_lists(psim_gpu_ctx* c) {
     }
     for (Live& L : c->live) {
         void* ls[] = { L.px, L.py, L.dx, L.dy, L.freq, L.vel, L.age, L.tts, L.rate0, L.rate1, L.rate2, L.cell, L.sensor,
-            L.meta, L.id };
+            L.meta, L.id, L.sid };
         for (void* p : ls) {
             if (p) { (void)hipFree(p); }
         }
@@ -892,8 +917,9 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
         c->pool.cell = dalloc<int32_t>(cap);
         c->pool.sensor = dalloc<int32_t>(cap);
         c->pool.meta = dalloc<uint32_t>(cap);
+        c->pool.sid = dalloc<uint32_t>(cap);
         if (!c->pool.px || !c->pool.py || !c->pool.dx || !c->pool.dy || !c->pool.freq || !c->pool.vel || !c->pool.age
-            || !c->pool.tts || !c->pool.cell || !c->pool.sensor || !c->pool.meta) {
+            || !c->pool.tts || !c->pool.cell || !c->pool.sensor || !c->pool.meta || !c->pool.sid) {
             free_pool(c);
             return fail(c, PSIM_E_DEVICE, "hipMalloc failed for the phonon pool");
         }
@@ -937,8 +963,9 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
             L.sensor = dalloc<int32_t>(cap);
             L.meta = dalloc<uint32_t>(cap);
             L.id = dalloc<uint32_t>(cap);
+            L.sid = dalloc<uint32_t>(cap);
             ok = ok && L.px && L.py && L.dx && L.dy && L.freq && L.vel && L.age && L.tts && L.rate0 && L.rate1 && L.rate2
-                 && L.cell && L.sensor && L.meta && L.id;
+                 && L.cell && L.sensor && L.meta && L.id && L.sid;
         }
         if (!ok) {
             free_lists(c);
@@ -947,7 +974,8 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
         c->list_cap = (int64_t)cap;
     }
     const Fate fate = c->fate_enabled ? c->fate : Fate{ nullptr, nullptr, nullptr };
-    const int32_t W = record_window_for(c);
+    const bool global_tally = LDS_TALLY_BYTES / (20 * std::max(1, c->n_sensors)) < GLOBAL_TALLY_STEPS;
+    const int32_t W = global_tally ? free_window_for(c) : record_window_for(c);
     const int64_t N = P.num_measurements;
     const auto windows = plan_windows(N, P.step_adjustment, free_window_for(c), W);
     const int64_t n_windows = (int64_t)windows.size();
@@ -966,7 +994,7 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
         }
         c->steps_cap = steps_need;
     }
-    const int64_t WS = (int64_t)W * S;
+    const int64_t WS = global_tally ? 0 : (int64_t)W * S;
     if (c->partial_cap < grid * WS) {
         free_partials(c);
         const size_t cap = (size_t)(grid * WS);
@@ -1008,14 +1036,14 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
             c->d_born_offset, w };
         // window steps k0+1 .. k1 map to tally indices (k - adj) in [0, R)
         const bool rec = (k1 >= P.step_adjustment) && (k0 + 1 - P.step_adjustment < R) && (k1 - k0 <= W);
-        Tally t{ c->d_pcnt, c->d_pfx, c->d_pfy, rec ? 1 : 0, W };
-        const size_t lds = rec ? (size_t)WS * 20 : 0;
+        Tally t{ c->d_pcnt, c->d_pfx, c->d_pfy, c->d_counts, c->d_flux, rec ? 1 : 0, global_tally ? 1 : 0, W };
+        const size_t lds = (rec && !global_tally) ? (size_t)WS * 20 : 0;
         HIP_CHECK(c, hipEventRecord(c->ev[1], c->stream));
         hipLaunchKernelGGL(transport_kernel, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool, lists, (int32_t)k0,
             (int32_t)k1, seed, t, c->d_counters, fate);
         HIP_CHECK(c, hipGetLastError());
         HIP_CHECK(c, hipEventRecord(c->ev[2], c->stream));
-        if (rec) {
+        if (rec && !global_tally) {
             const int rblocks = (int)((WS + BLOCK - 1) / BLOCK);
             hipLaunchKernelGGL(reduce_kernel, dim3(rblocks), dim3(BLOCK), 0, c->stream, c->d_pcnt, c->d_pfx, c->d_pfy,
                 grid, (int)WS, S, k0, P.step_adjustment, R, c->d_counts, c->d_flux);
@@ -1138,8 +1166,8 @@ int psim_gpu_upload(psim_gpu_ctx* c, const psim_geom* g, const psim_tables* t, c
         || p->total_phonons > (int64_t)UINT32_MAX) {
         return fail(c, PSIM_E_INVALID, "bad run parameters");
     }
-    if (g->n_sensors > 3200) {
-        return fail(c, PSIM_E_INVALID, "at most 3200 sensors (one window step of tallies must fit in LDS)");
+    if ((int64_t)g->n_sensors * p->num_record >= ((int64_t)1 << 40)) {
+        return fail(c, PSIM_E_INVALID, "sensors x recorded steps too large");
     }
     // Every index the kernels follow is checked here, so a bad descriptor cannot make them
     // read outside the uploaded arrays.
@@ -1174,7 +1202,9 @@ int psim_gpu_upload(psim_gpu_ctx* c, const psim_geom* g, const psim_tables* t, c
     for (int i = 0; i < s->n_sources; ++i) {
         const psim_source& src = s->sources[i];
         if (src.cell < 0 || src.cell >= g->n_cells || src.table < 0 || src.table >= t->n_tables || src.count < 0
-            || src.offset != expect || (src.kind != 0 && src.kind != 1)) {
+            || src.offset != expect || (src.kind != 0 && src.kind != 1) || src.stream < 0 || src.stream > 0xffff
+            || src.id_base < 0 || src.id_base > src.offset
+            || src.offset + src.count - src.id_base > (int64_t)UINT32_MAX) {
             return fail(c, PSIM_E_INVALID, "bad emission source");
         }
         expect += src.count;

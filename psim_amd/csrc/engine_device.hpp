@@ -33,11 +33,13 @@ constexpr int GUIDE = 2048;  // guide-table cells per frequency table (a power o
 enum : int { POL_LA = 0, POL_TA = 1 };
 enum : int { FATE_ABSORBED = 1, FATE_EXPIRED = 2, FATE_NOT_STARTED = 3, FATE_STALLED = 4 };
 
-// Phonon record word: bits 0-9 frequency bin, bit 10 TA, bit 11 negative sign, bit 12 started.
+// Phonon record word: bits 0-9 frequency bin, bit 10 TA, bit 11 negative sign, bit 12 started,
+// bits 16-31 random-stream index (the model's index in a batch, 0 otherwise).
 constexpr uint32_t META_BIN_MASK = 0x3ffu;
 constexpr uint32_t META_TA = 1u << 10;
 constexpr uint32_t META_NEG = 1u << 11;
 constexpr uint32_t META_STARTED = 1u << 12;
+constexpr int META_STREAM_SHIFT = 16;
 
 struct Model {
     const psim_cell* __restrict__ cells;
@@ -78,6 +80,7 @@ struct Pool {
     int32_t* __restrict__ cell;
     int32_t* __restrict__ sensor;  // sensor of the cell
     uint32_t* __restrict__ meta;
+    uint32_t* __restrict__ sid;  // id within the phonon's random stream (id - source id_base)
 };
 
 struct Ph {

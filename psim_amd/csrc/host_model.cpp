@@ -4,6 +4,7 @@
 #include "host_model.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -13,6 +14,7 @@
 #include <numeric>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 
 #include "json.hpp"
 #include "psim_rng.h"
@@ -455,7 +457,7 @@ void GpuModelSimulator::initPhononBuilders(std::vector<Cell>& cells, std::vector
         const double frac = std::modf(energy / eff_energy, &ipart);
         int64_t n = static_cast<int64_t>(ipart);
         psim_stream st;
-        psim_stream_init(&st, seed_, PSIM_TAG_BUDGET, budget_index++, 0);
+        psim_stream_init(&st, seed_, psim_stream_tag(PSIM_TAG_BUDGET, (uint32_t)stream_), budget_index++, 0);
         if (psim_urand(&st) < frac) { ++n; }
         return n;
     };
@@ -473,6 +475,7 @@ void GpuModelSimulator::initPhononBuilders(std::vector<Cell>& cells, std::vector
             src.sign = (cell.sensor().getInitTemp() > t_eq) ? 1 : -1;
             src.offset = offset;
             src.count = n;
+            src.stream = stream_;
             offset += n;
             d_sources.push_back(src);
         }
@@ -534,6 +537,7 @@ void GpuModelSimulator::initPhononBuilders(std::vector<Cell>& cells, std::vector
                 src.sign = (es.temp > t_eq) ? 1 : -1;
                 src.offset = offset;
                 src.count = get_phonons(energy);
+                src.stream = stream_;
                 offset += src.count;
                 src.start = es.start;
                 src.duration = es.duration;
@@ -563,11 +567,6 @@ void GpuModelSimulator::initPhononBuilders(std::vector<Cell>& cells, std::vector
         frac.insert(frac.end(), t.la_frac.begin(), t.la_frac.end());
     }
     const bool full = !mats.empty() && mats.front()->full();
-    psim_geom g{ static_cast<int32_t>(cells.size()), static_cast<int32_t>(sensors.size()),
-        static_cast<int32_t>(d_subs.size()), 0, d_cells.data(), d_edges.data(), d_subs.data(), d_sensors.data() };
-    psim_tables t{ static_cast<int32_t>(d_mats.size()), static_cast<int32_t>(table_keys.size()), d_mats.data(),
-        freq.data(), vel_la.data(), vel_ta.data(), cdf.data(), frac.data() };
-    psim_sources s{ static_cast<int32_t>(d_sources.size()), 0, d_sources.data() };
     psim_run_params p{};
     p.num_measurements = static_cast<int64_t>(measurement_steps_);
     p.step_adjustment = static_cast<int64_t>(step_adjustment_);
@@ -580,7 +579,16 @@ void GpuModelSimulator::initPhononBuilders(std::vector<Cell>& cells, std::vector
     p.phasor = phasor_sim_ ? 1 : 0;
     source_counts_.clear();
     for (const auto& src : d_sources) { source_counts_.push_back(src.count); }
+    flat_ = FlatModel{ std::move(d_cells), std::move(d_edges), std::move(d_subs), std::move(d_sensors), std::move(d_mats),
+        std::move(freq), std::move(vel_la), std::move(vel_ta), std::move(cdf), std::move(frac), std::move(d_sources), p,
+        static_cast<int32_t>(table_keys.size()) };
     if (!ctx_) { return; }  // budget only (no engine attached)
+    const FlatModel& f = flat_;
+    psim_geom g{ static_cast<int32_t>(f.cells.size()), static_cast<int32_t>(f.sensors.size()),
+        static_cast<int32_t>(f.subs.size()), 0, f.cells.data(), f.edges.data(), f.subs.data(), f.sensors.data() };
+    psim_tables t{ static_cast<int32_t>(f.mats.size()), f.n_tables, f.mats.data(), f.freq.data(), f.vel_la.data(),
+        f.vel_ta.data(), f.cdf.data(), f.frac.data() };
+    psim_sources s{ static_cast<int32_t>(f.sources.size()), 0, f.sources.data() };
     if (psim_gpu_upload(ctx_, &g, &t, &s, &p) != PSIM_OK) {
         throw std::runtime_error(std::string("engine upload failed: ") + psim_gpu_last_error(ctx_));
     }
@@ -828,42 +836,171 @@ void Model::runSimulation(psim_gpu_ctx* ctx, uint64_t seed) {  // model.cpp:141-
         simulator_.setEngine(ctx);
         const double eff0 = prepare(seed + run * RUN_SEED_STRIDE);
         simulator_.runSimulation(t_eq_, sensors_);
-        RunRecord rec;
-        rec.eff_energy = eff0;
-        rec.total_phonons = simulator_.totalPhonons();
-        rec.counters = simulator_.counters();
-        const std::size_t R = num_record_;
-        for (const auto& s : sensors_) {
-            rec.counts.insert(rec.counts.end(), s.inc_energy.begin(), s.inc_energy.end());
-            for (const auto& f : s.inc_flux) {
-                rec.flux.push_back(f[0]);
-                rec.flux.push_back(f[1]);
-            }
+        endRun(eff0, simulator_.totalPhonons(), simulator_.counters());
+    }
+}
+
+void Model::endRun(double eff0, int64_t total_phonons, const std::array<int64_t, PSIM_NUM_COUNTERS>& counters) {
+    RunRecord rec;
+    rec.eff_energy = eff0;
+    rec.total_phonons = total_phonons;
+    rec.counters = counters;
+    const std::size_t R = num_record_;
+    for (const auto& s : sensors_) {
+        rec.counts.insert(rec.counts.end(), s.inc_energy.begin(), s.inc_energy.end());
+        for (const auto& f : s.inc_flux) {
+            rec.flux.push_back(f[0]);
+            rec.flux.push_back(f[1]);
         }
-        // Model::resetRequired (model.cpp:250-272): getFinalTemp with the budget's energy sets
-        // each controller's t_steady_; refresh() then recomputes the energy per phonon with it.
-        if (sim_type_ != SimulationType::Transient) {
-            for (auto& s : sensors_) {
-                double ft = 0.;
-                if (s.getArea() != 0.) {
-                    const auto temps = findTemperature(s, eff0, start_step_);
-                    ft = std::accumulate(temps.begin(), temps.end(), 0.) / static_cast<double>(R - start_step_);
-                }
-                s.setSteadyTemp(ft);
-            }
-        }
-        const double eff1 = getTotalInitialEnergy() / static_cast<double>(num_phonons_);
-        for (const auto& s : sensors_) { rec.measurements.push_back(scaleHeatParams(s, eff1)); }
-        std::sort(rec.measurements.begin(), rec.measurements.end(),
-            [](const auto& a, const auto& b) { return a.id < b.id; });
-        runs_.push_back(std::move(rec));
-        // Model::reset(true): controllers back to t_init, tallies cleared
+    }
+    // Model::resetRequired (model.cpp:250-272): getFinalTemp with the budget's energy sets
+    // each controller's t_steady_; refresh() then recomputes the energy per phonon with it.
+    if (sim_type_ != SimulationType::Transient) {
         for (auto& s : sensors_) {
-            s.setSteadyTemp(s.t_init());
-            std::fill(s.inc_energy.begin(), s.inc_energy.end(), 0);
-            std::fill(s.inc_flux.begin(), s.inc_flux.end(), std::array<double, 2>{ 0., 0. });
+            double ft = 0.;
+            if (s.getArea() != 0.) {
+                const auto temps = findTemperature(s, eff0, start_step_);
+                ft = std::accumulate(temps.begin(), temps.end(), 0.) / static_cast<double>(R - start_step_);
+            }
+            s.setSteadyTemp(ft);
         }
-        simulator_.reset();
+    }
+    const double eff1 = getTotalInitialEnergy() / static_cast<double>(num_phonons_);
+    for (const auto& s : sensors_) { rec.measurements.push_back(scaleHeatParams(s, eff1)); }
+    std::sort(rec.measurements.begin(), rec.measurements.end(),
+        [](const auto& a, const auto& b) { return a.id < b.id; });
+    runs_.push_back(std::move(rec));
+    // Model::reset(true): controllers back to t_init, tallies cleared
+    for (auto& s : sensors_) {
+        s.setSteadyTemp(s.t_init());
+        std::fill(s.inc_energy.begin(), s.inc_energy.end(), 0);
+        std::fill(s.inc_flux.begin(), s.inc_flux.end(), std::array<double, 2>{ 0., 0. });
+    }
+    simulator_.reset();
+}
+
+void Model::runBatch(const std::vector<Model*>& models, psim_gpu_ctx* ctx, uint64_t seed,
+    std::array<int64_t, PSIM_NUM_COUNTERS>& counters) {
+    if (!ctx) { throw std::runtime_error("no engine context"); }
+    if (models.empty()) { throw std::runtime_error("empty batch"); }
+    if (models.size() > 65536) { throw std::runtime_error("at most 65536 models per batch"); }
+    const Model& m0 = *models.front();
+    for (const Model* m : models) {
+        if (!m) { throw std::runtime_error("null model in batch"); }
+        if (m->measurement_steps_ != m0.measurement_steps_ || m->simulation_time_ != m0.simulation_time_
+            || m->sim_type_ != m0.sim_type_ || m->step_interval_ != m0.step_interval_
+            || m->phasor_sim_ != m0.phasor_sim_ || m->num_runs_ != 1) {
+            throw std::runtime_error("batched models must share num_measurements, sim_time, sim_type, step_interval "
+                                     "and phasor_sim, and have num_runs 1");
+        }
+    }
+    // Each model's pre-run state and budget, exactly as its single run with stream m builds them.
+    // The models share nothing, so their tables and budgets are built on a few host threads.
+    std::vector<double> eff0(models.size());
+    std::vector<int32_t> own_stream(models.size());
+    std::vector<std::string> errors(models.size());
+    std::atomic<std::size_t> next{ 0 };
+    auto work = [&]() {
+        for (std::size_t i = next++; i < models.size(); i = next++) {
+            Model& m = *models[i];
+            try {
+                own_stream[i] = m.simulator_.stream();
+                m.runs_.clear();
+                m.simulator_.setEngine(nullptr);
+                m.simulator_.setStream(static_cast<int32_t>(i));
+                eff0[i] = m.prepare(seed);
+            } catch (const std::exception& e) { errors[i] = e.what(); }
+        }
+    };
+    const std::size_t n_threads = std::min<std::size_t>(models.size(),
+        std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (std::size_t t = 1; t < n_threads; ++t) { pool.emplace_back(work); }
+    work();
+    for (auto& th : pool) { th.join(); }
+    for (const auto& e : errors) {
+        if (!e.empty()) { throw std::runtime_error(e); }
+    }
+    // One set of descriptors: every index shifted by the sizes of the models before it.
+    FlatModel all;
+    std::vector<int64_t> first_id(models.size()), sensor_base(models.size());
+    for (std::size_t i = 0; i < models.size(); ++i) {
+        const FlatModel& f = models[i]->simulator_.flat();
+        if (f.params.full != m0.simulator_.flat().params.full) {
+            throw std::runtime_error("batched models must all be full or all deviational");
+        }
+        const int32_t cell0 = static_cast<int32_t>(all.cells.size());
+        const int32_t sensor0 = static_cast<int32_t>(all.sensors.size());
+        const int32_t sub0 = static_cast<int32_t>(all.subs.size());
+        const int32_t mat0 = static_cast<int32_t>(all.mats.size());
+        const int32_t table0 = all.n_tables;
+        const int64_t id0 = all.params.total_phonons;
+        first_id[i] = id0;
+        sensor_base[i] = sensor0;
+        for (psim_cell c : f.cells) {
+            c.sensor += sensor0;
+            all.cells.push_back(c);
+        }
+        for (psim_edge e : f.edges) {
+            e.sub_begin += sub0;
+            all.edges.push_back(e);
+        }
+        for (psim_subsurface ss : f.subs) {
+            if (ss.kind == 0) { ss.target += cell0; }
+            all.subs.push_back(ss);
+        }
+        for (psim_sensor se : f.sensors) {
+            se.material += mat0;
+            se.base_table += table0;
+            se.scatter_table += table0;
+            all.sensors.push_back(se);
+        }
+        all.mats.insert(all.mats.end(), f.mats.begin(), f.mats.end());
+        all.freq.insert(all.freq.end(), f.freq.begin(), f.freq.end());
+        all.vel_la.insert(all.vel_la.end(), f.vel_la.begin(), f.vel_la.end());
+        all.vel_ta.insert(all.vel_ta.end(), f.vel_ta.begin(), f.vel_ta.end());
+        all.cdf.insert(all.cdf.end(), f.cdf.begin(), f.cdf.end());
+        all.frac.insert(all.frac.end(), f.frac.begin(), f.frac.end());
+        all.n_tables += f.n_tables;
+        for (psim_source src : f.sources) {
+            src.cell += cell0;
+            src.table += table0;
+            src.offset += id0;
+            src.id_base = id0;
+            all.sources.push_back(src);
+        }
+        all.params.total_phonons += f.params.total_phonons;
+    }
+    psim_run_params p = m0.simulator_.flat().params;
+    p.total_phonons = all.params.total_phonons;
+    psim_geom g{ static_cast<int32_t>(all.cells.size()), static_cast<int32_t>(all.sensors.size()),
+        static_cast<int32_t>(all.subs.size()), 0, all.cells.data(), all.edges.data(), all.subs.data(),
+        all.sensors.data() };
+    psim_tables t{ static_cast<int32_t>(all.mats.size()), all.n_tables, all.mats.data(), all.freq.data(),
+        all.vel_la.data(), all.vel_ta.data(), all.cdf.data(), all.frac.data() };
+    psim_sources s{ static_cast<int32_t>(all.sources.size()), 0, all.sources.data() };
+    if (psim_gpu_upload(ctx, &g, &t, &s, &p) != PSIM_OK) {
+        throw std::runtime_error(std::string("engine upload failed: ") + psim_gpu_last_error(ctx));
+    }
+    const std::size_t S = all.sensors.size(), R = m0.num_record_;
+    std::vector<int64_t> counts(S * R);
+    std::vector<double> flux(2 * S * R);
+    if (psim_gpu_run(ctx, seed, counts.data(), flux.data(), counters.data()) != PSIM_OK) {
+        throw std::runtime_error(std::string("engine run failed: ") + psim_gpu_last_error(ctx));
+    }
+    // Each model's slice of the tallies into its sensors, then the rest of its run.
+    const std::array<int64_t, PSIM_NUM_COUNTERS> none{};
+    for (std::size_t i = 0; i < models.size(); ++i) {
+        Model& m = *models[i];
+        for (std::size_t si = 0; si < m.sensors_.size(); ++si) {
+            const std::size_t row = static_cast<std::size_t>(sensor_base[i]) + si;
+            for (std::size_t k = 0; k < R; ++k) {
+                m.sensors_[si].inc_energy[k] = counts[row * R + k];
+                m.sensors_[si].inc_flux[k] = { flux[2 * (row * R + k)], flux[2 * (row * R + k) + 1] };
+            }
+        }
+        m.endRun(eff0[i], m.simulator_.flat().params.total_phonons, none);
+        m.simulator_.setStream(own_stream[i]);
     }
 }
 

@@ -192,6 +192,19 @@ struct RunRecord {
     int64_t total_phonons{ 0 };
 };
 
+// A model flattened into the engine's descriptors (include/psim_gpu.h).
+struct FlatModel {
+    std::vector<psim_cell> cells;
+    std::vector<psim_edge> edges;
+    std::vector<psim_subsurface> subs;
+    std::vector<psim_sensor> sensors;
+    std::vector<psim_material> mats;
+    std::vector<double> freq, vel_la, vel_ta, cdf, frac;
+    std::vector<psim_source> sources;
+    psim_run_params params{};
+    int32_t n_tables = 0;
+};
+
 // Drop-in for ModelSimulator (modelSimulator.h:12-41): the same two calls, backed by the engine.
 class GpuModelSimulator {
 public:
@@ -200,6 +213,9 @@ public:
     void setSeed(uint64_t seed) noexcept { seed_ = seed; }
     void setStepAdjustment(std::size_t adj) noexcept { step_adjustment_ = adj; }
     void setRecordLength(std::size_t r) noexcept { num_record_ = r; }
+    // Random-stream index of this model's phonons (0 unless it is model m of a batch).
+    void setStream(int32_t stream) noexcept { stream_ = stream; }
+    int32_t stream() const noexcept { return stream_; }
     // ModelSimulator::initPhononBuilders (modelSimulator.cpp:43-85): the emission budget per
     // source (Bernoulli rounding drawn from the Philox budget stream), then the flattened model
     // is uploaded to the engine.
@@ -211,9 +227,12 @@ public:
     int64_t totalPhonons() const noexcept { return total_phonons_; }
     const std::array<int64_t, PSIM_NUM_COUNTERS>& counters() const noexcept { return counters_; }
     const std::vector<int64_t>& sourceCounts() const noexcept { return source_counts_; }
+    const FlatModel& flat() const noexcept { return flat_; }
 
 private:
     psim_gpu_ctx* ctx_ = nullptr;
+    FlatModel flat_;
+    int32_t stream_ = 0;
     std::vector<int64_t> source_counts_;
     uint64_t seed_ = 0;
     std::size_t measurement_steps_;
@@ -237,6 +256,12 @@ public:
     // Model::runSimulation (model.cpp:141-182) with the engine of `ctx`; run r uses Philox seed
     // seed + r * 0x9E3779B97F4A7C15.
     void runSimulation(psim_gpu_ctx* ctx, uint64_t seed);
+    // Runs the models together: one engine upload and one run (psim_batch_run).  Model m draws the
+    // random numbers of its single run with stream m; each keeps its own budget and results.
+    // `counters` receives the batch totals.
+    static void runBatch(const std::vector<Model*>& models, psim_gpu_ctx* ctx, uint64_t seed,
+        std::array<int64_t, PSIM_NUM_COUNTERS>& counters);
+    void setStream(int32_t stream) noexcept { simulator_.setStream(stream); }
     void exportResults(const std::filesystem::path& filepath, double time) const;
     // Pre-run state of one run (bounds, tables, energy per phonon, emission budget) without an
     // engine; returns the budget's energy per phonon.
@@ -269,6 +294,8 @@ private:
 
     Sensor& getSensor(std::size_t id);
     double prepare(uint64_t seed);
+    // The rest of one run once the sensors hold its tallies (model.cpp:161-181).
+    void endRun(double eff0, int64_t total_phonons, const std::array<int64_t, PSIM_NUM_COUNTERS>& counters);
     double getTotalInitialEnergy() const;
     std::pair<double, double> setTemperatureBounds();
     std::vector<double> findTemperature(const Sensor& s, double eff_energy, std::size_t start) const;

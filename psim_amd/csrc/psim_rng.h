@@ -14,7 +14,7 @@
  *   c0 = block index j of the stream (each block yields two doubles)
  *   c1 = measurement step for PSIM_TAG_TRANSPORT, 0 otherwise
  *   c2 = phonon id (transport, emission) or source index (budget)
- *   c3 = stream tag
+ *   c3 = stream tag | stream index << 8 (the index is 0 except for batched models)
  * Draw n of a stream is the (n % 2)-th double of block n / 2.  A transport
  * stream restarts at draw 0 whenever the phonon's measurement step advances.
  *
@@ -93,6 +93,9 @@ PSIM_QUAL void psim_stream_init(psim_stream* s, uint64_t seed, uint32_t tag, uin
     s->spare = 0.0;
     s->has_spare = 0;
 }
+
+/* Counter word c3 of a stream: its tag plus a stream index (a model's index in a batch). */
+PSIM_QUAL uint32_t psim_stream_tag(uint32_t tag, uint32_t stream) { return tag | (stream << 8); }
 
 /* Restart the stream for a new measurement step (transport streams). */
 PSIM_QUAL void psim_stream_set_step(psim_stream* s, uint32_t step) {

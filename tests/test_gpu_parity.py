@@ -16,9 +16,10 @@ from psim_amd import configs
 pytestmark = pytest.mark.gpu
 
 
-def _bar(spec, sim_type=0, step_interval=0, n=20000, steps=1000, t_eq=300.0):
-    return configs.linear_bar(num_phonons=n, num_measurements=steps, sim_time=steps * 0.005, spec=spec,
-                              sim_type=sim_type, step_interval=step_interval, t_eq=t_eq).to_dict()
+def _bar(spec, sim_type=0, step_interval=0, n=20000, steps=1000, t_eq=300.0, sensors=10):
+    return configs.linear_bar(num_sensors=sensors, length=100.0 * sensors, num_phonons=n, num_measurements=steps,
+                              sim_time=steps * 0.005, spec=spec, sim_type=sim_type, step_interval=step_interval,
+                              t_eq=t_eq).to_dict()
 
 
 CASES = {
@@ -26,6 +27,8 @@ CASES = {
     "bar_ss_specular": _bar(1.0),
     "bar_periodic_mixed": _bar(0.5, sim_type=1, step_interval=50),
     "bar_full_mode": _bar(0.2, t_eq=0.0),
+    # 400 sensors: a window's tallies no longer fit LDS, so they go to HBM with atomics
+    "bar_400_sensors_periodic": _bar(0.4, sim_type=1, step_interval=50, n=30000, sensors=400),
     "lshape_full": configs.lshape_full(num_phonons=20000, num_measurements=1000),
     "film_transient": configs.film_transient(num_phonons=20000, num_measurements=1000, sim_time=1.0,
                                              step_interval=50),
