@@ -211,6 +211,21 @@ int psim_gpu_last_timing(const psim_gpu_ctx* ctx, double* step_kernel_ms, int64_
 /* Engine tuning: measurement steps per transport launch (0 = automatic). */
 int psim_gpu_set_window(psim_gpu_ctx* ctx, int32_t steps_per_launch);
 
+/* Engine tuning: transport blocks per compute unit (default 8). */
+int psim_gpu_set_grid(psim_gpu_ctx* ctx, int32_t blocks_per_cu);
+
+/* psim_gpu_run replayed from a HIP graph of the whole run (emission, sorts, every transport
+ * window and reduction), captured on the first call and again after anything that changes the
+ * launch sequence (upload, window, grid, fate recording).  The seed is data, so one graph serves
+ * every seed.  Same results as psim_gpu_run; no counterpart in the reference (its loop is a
+ * std::for_each, modelSimulator.cpp:227-253). */
+int psim_gpu_run_graph(psim_gpu_ctx* ctx, uint64_t seed, int64_t* counts, double* flux, int64_t* counters);
+
+/* Autotuner: runs the uploaded model once for each candidate launch shape (untallied window
+ * length 0 / 1024 / 256, 8 or 3 transport blocks per CU; none changes any result), keeps the
+ * fastest in the context and reports it.  No counterpart in the reference. */
+int psim_gpu_autotune(psim_gpu_ctx* ctx, uint64_t seed, int32_t* window, int32_t* blocks_per_cu, double* best_ms);
+
 /* ---- host model (C++17 behind the C ABI) ------------------------------------ */
 
 typedef struct psim_model psim_model;

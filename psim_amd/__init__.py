@@ -67,6 +67,11 @@ def lib():
         "psim_gpu_fetch_fate": (ctypes.c_int, [_VP, _I64, _I32, _D]),
         "psim_gpu_last_timing": (ctypes.c_int, [_VP, _D, _I64, _D]),
         "psim_gpu_set_window": (ctypes.c_int, [_VP, ctypes.c_int32]),
+        "psim_gpu_set_grid": (ctypes.c_int, [_VP, ctypes.c_int32]),
+        "psim_gpu_run_graph": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64),
+                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
+        "psim_gpu_autotune": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32),
+                                             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
         "psim_model_from_json": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
         "psim_model_destroy": (None, [_VP]),
         "psim_model_last_error": (ctypes.c_char_p, [_VP]),
@@ -134,8 +139,28 @@ class Engine:
     def set_window(self, steps: int) -> None:
         self._check(self._L.psim_gpu_set_window(self._h, int(steps)))
 
+    def set_grid(self, blocks_per_cu: int) -> None:
+        self._check(self._L.psim_gpu_set_grid(self._h, int(blocks_per_cu)))
+
     def run_resident(self, seed: int) -> None:
         self._check(self._L.psim_gpu_run_resident(self._h, int(seed)))
+
+    def run_graph(self, n_sensors: int, num_record: int, seed: int):
+        """The uploaded model's run replayed from a captured HIP graph: (counts, flux, counters)."""
+        counts = np.zeros((n_sensors, num_record), np.int64)
+        flux = np.zeros((n_sensors, num_record, 2), np.float64)
+        counters = np.zeros(len(COUNTER_NAMES), np.int64)
+        self._check(self._L.psim_gpu_run_graph(self._h, int(seed), _p(counts, _I64), _p(flux, _D),
+                                               _p(counters, _I64)))
+        return counts, flux, dict(zip(COUNTER_NAMES, counters.tolist()))
+
+    def autotune(self, seed: int = 0) -> dict:
+        """Tries the launch shapes of psim_gpu_autotune on the uploaded model and keeps the fastest."""
+        w = ctypes.c_int32()
+        g = ctypes.c_int32()
+        ms = ctypes.c_double()
+        self._check(self._L.psim_gpu_autotune(self._h, int(seed), ctypes.byref(w), ctypes.byref(g), ctypes.byref(ms)))
+        return {"window": w.value, "blocks_per_cu": g.value, "run_ms": ms.value}
 
     def fetch(self, n_sensors: int, num_record: int):
         counts = np.zeros((n_sensors, num_record), np.int64)

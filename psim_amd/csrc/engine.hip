@@ -80,9 +80,10 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 
 // Phonon builders, psim/src/phononBuilder.cpp:6-49, with EmitSurface::getPhononTime
 // (surface.cpp:67-69) and Line/Triangle::getRandPoint (geometry.cpp:140-143, 234-242).
-__global__ void __launch_bounds__(BLOCK) emit_kernel(Model m, Pool pool, int64_t total, uint64_t seed,
-    unsigned long long* __restrict__ counters, Fate fate, int32_t* __restrict__ birth_key,
-    uint32_t* __restrict__ birth_hist) {
+__global__ void __launch_bounds__(BLOCK) emit_kernel(Model m, Pool pool, int64_t total,
+    const uint64_t* __restrict__ seed_p, unsigned long long* __restrict__ counters, Fate fate,
+    int32_t* __restrict__ birth_key, uint32_t* __restrict__ birth_hist) {
+    const uint64_t seed = *seed_p;
     const int64_t id = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     uint32_t not_started = 0;
     int32_t key = -1;
@@ -270,12 +271,16 @@ enum : int { S_FETCH = 0, S_TTS = 1, S_NI = 2, S_COLL = 3, S_SCAT = 4, S_EXIT = 
 // scattering-time draw) run once for many lanes instead of once per lane that needs them.
 // Each lane still performs exactly the reference's sequence of operations for its phonon, so
 // trajectories are unchanged.
+// GLOBAL_TALLY: compile-time choice of the tally path (Tally::global), so that the common LDS
+// path carries no code of the other.
+template <bool GLOBAL_TALLY>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PSIM_TRANSPORT_WAVES_PER_EU)))
-transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64_t seed, Tally tally,
-    unsigned long long* __restrict__ counters, Fate fate) {
+transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const uint64_t* __restrict__ seed_p,
+    Tally tally, unsigned long long* __restrict__ counters, Fate fate) {
+    const uint64_t seed = *seed_p;
     extern __shared__ unsigned char lds_raw[];
     const int S = m.n_sensors;
-    const int WS = (tally.rec && !tally.global) ? tally.W * S : 0;
+    const int WS = (tally.rec && !GLOBAL_TALLY) ? tally.W * S : 0;
     double* lfx = reinterpret_cast<double*>(lds_raw);
     double* lfy = lfx + WS;
     int32_t* lcnt = reinterpret_cast<int32_t*>(lfy + WS);
@@ -296,7 +301,6 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
 
     int state = S_FETCH;
     uint32_t slot = 0;
-    uint32_t sid = 0, stream = 0;  // the phonon's random stream: (stream index, id within it)
     Ph p;
     p.px = p.py = p.dx = p.dy = p.freq = p.vel = 0.;
     p.cell = p.bin = p.pol = 0;
@@ -356,7 +360,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
                     const int idx = p.step - m.adj;
                     if (idx < m.R && tally.rec) {
                         const double sg = (double)p.sign;
-                        if (tally.global) {
+                        if constexpr (GLOBAL_TALLY) {
                             const int64_t t = (int64_t)cur_sensor * m.R + idx;
                             atomicAdd(reinterpret_cast<unsigned long long*>(&tally.counts[t]),
                                 (unsigned long long)(int64_t)p.sign);
@@ -436,8 +440,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
                     slot = started ? L.id[item] : lists.born_order[b0 + (item - n_live)];
                     const int64_t j = started ? item : (int64_t)slot;
                     const uint32_t meta = (started ? L.meta : pool.meta)[j];
-                    sid = (started ? L.sid : pool.sid)[j];
-                    stream = meta >> META_STREAM_SHIFT;
+                    const uint32_t sid = (started ? L.sid : pool.sid)[j];
                     p.cell = (started ? L.cell : pool.cell)[j];
                     cur_sensor = (started ? L.sensor : pool.sensor)[j];
                     p.px = (started ? L.px : pool.px)[j];
@@ -459,7 +462,9 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
                     p.step = started ? k0 : (int32_t)(age / m.step_time);
                     ttm = 0.;
                     events = 0;
-                    psim_stream_init(&st, seed, psim_stream_tag(PSIM_TAG_TRANSPORT, stream), sid, (uint32_t)p.step);
+                    // the stream keeps (stream index, id within it) in its counter words c3 and c2
+                    psim_stream_init(&st, seed, psim_stream_tag(PSIM_TAG_TRANSPORT, meta >> META_STREAM_SHIFT), sid,
+                        (uint32_t)p.step);
                     begin_event();
                 }
             }
@@ -554,9 +559,9 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, uint64
                 O.cell[k] = p.cell;
                 O.sensor[k] = cur_sensor;
                 O.meta[k] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
-                            | META_STARTED | (stream << META_STREAM_SHIFT);
+                            | META_STARTED | ((st.c3 >> 8) << META_STREAM_SHIFT);
                 O.id[k] = slot;
-                O.sid[k] = sid;
+                O.sid[k] = st.c2;
                 state = S_FETCH;
             }
         }
@@ -695,6 +700,11 @@ __global__ void __launch_bounds__(SORT_BLOCK) cell_scatter_kernel(Live src, Live
     }
 }
 
+// Clears n 32-bit words (used instead of memset nodes, so that a run can be captured in a graph).
+__global__ void __launch_bounds__(BLOCK) zero_words_kernel(uint32_t* __restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) { p[i] = 0u; }
+}
+
 // Sums the block partials of one window into the tallies: entry e = w * S + s of window step
 // k0 + 1 + w goes to tally index (k0 + 1 + w) - adj of sensor s.
 __global__ void __launch_bounds__(BLOCK) reduce_kernel(const int32_t* __restrict__ pcnt,
@@ -768,6 +778,14 @@ struct psim_gpu_ctx {
     uint32_t* d_sort_keys = nullptr;  // [SORT_KEYS] counts, then cursors, of the cell sort
     int64_t steps_cap = 0;
     int32_t window = 0;
+    int32_t blocks_per_cu = 8;  // transport grid: n_CU x blocks_per_cu blocks
+    // seed of the current run, read by the kernels from device memory (pinned host staging copy)
+    uint64_t* d_seed = nullptr;
+    uint64_t* h_seed = nullptr;
+    // captured run (psim_gpu_run_graph); valid while `gen` (bumped by anything that changes the
+    // launch sequence or the buffers it names) equals graph_gen
+    hipGraphExec_t graph_exec = nullptr;
+    uint64_t gen = 1, graph_gen = 0;
     // timing of the last run
     double step_ms = 0., run_ms = 0.;
     int64_t step_launches = 0;
@@ -895,7 +913,18 @@ std::vector<std::pair<int64_t, int64_t>> plan_windows(int64_t N, int64_t adj, in
     return out;
 }
 
-int run_impl(psim_gpu_ctx* c, uint64_t seed) {
+// Launch plan of one run of the uploaded model.
+struct RunPlan {
+    int64_t n = 0, N = 0, R = 0, blocks_needed = 0, steps_need = 0, WS = 0;
+    int S = 0, grid = 1;
+    int32_t W = 0;
+    bool global_tally = false;
+    std::vector<std::pair<int64_t, int64_t>> windows;
+};
+
+// Allocates (or reuses) every buffer the run needs and plans its windows.  Any reallocation
+// bumps c->gen, which invalidates a captured graph.
+int prepare_run(psim_gpu_ctx* c, RunPlan& plan) {
     if (!c->uploaded) { return fail(c, PSIM_E_STATE, "psim_gpu_upload must be called before running"); }
     HIP_CHECK(c, hipSetDevice(c->device));
     const psim_run_params& P = c->params;
@@ -924,6 +953,7 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
             return fail(c, PSIM_E_DEVICE, "hipMalloc failed for the phonon pool");
         }
         c->pool_cap = (int64_t)cap;
+        c->gen++;
     }
     if (c->fate_enabled && c->fate_cap < n) {
         free_fate(c);
@@ -936,9 +966,10 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
             return fail(c, PSIM_E_DEVICE, "hipMalloc failed for the fate arrays");
         }
         c->fate_cap = (int64_t)cap;
+        c->gen++;
     }
     const int64_t blocks_needed = (n + BLOCK - 1) / BLOCK;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks_needed, (int64_t)c->n_cu * 8));
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks_needed, (int64_t)c->n_cu * c->blocks_per_cu));
     // A live buffer holds every phonon plus the survivor slots the waves may reserve but not fill.
     const int64_t live_need = n + (int64_t)OUT_CLAIM * grid * (BLOCK / WAVE);
     if (c->list_cap < live_need) {
@@ -972,8 +1003,8 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
             return fail(c, PSIM_E_DEVICE, "hipMalloc failed for the phonon lists");
         }
         c->list_cap = (int64_t)cap;
+        c->gen++;
     }
-    const Fate fate = c->fate_enabled ? c->fate : Fate{ nullptr, nullptr, nullptr };
     const bool global_tally = LDS_TALLY_BYTES / (20 * std::max(1, c->n_sensors)) < GLOBAL_TALLY_STEPS;
     const int32_t W = global_tally ? free_window_for(c) : record_window_for(c);
     const int64_t N = P.num_measurements;
@@ -993,6 +1024,7 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
             return fail(c, PSIM_E_DEVICE, "hipMalloc failed for the step arrays");
         }
         c->steps_cap = steps_need;
+        c->gen++;
     }
     const int64_t WS = global_tally ? 0 : (int64_t)W * S;
     if (c->partial_cap < grid * WS) {
@@ -1006,17 +1038,52 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
             return fail(c, PSIM_E_DEVICE, "hipMalloc failed for tally partials");
         }
         c->partial_cap = (int64_t)cap;
+        c->gen++;
     }
-    HIP_CHECK(c, hipMemsetAsync(c->d_counts, 0, sizeof(int64_t) * (size_t)std::max<int64_t>(1, S * R), c->stream));
-    HIP_CHECK(c, hipMemsetAsync(c->d_flux, 0, sizeof(double) * (size_t)std::max<int64_t>(1, 2 * S * R), c->stream));
-    HIP_CHECK(c, hipMemsetAsync(c->d_counters, 0, sizeof(unsigned long long) * PSIM_NUM_COUNTERS, c->stream));
-    HIP_CHECK(c, hipMemsetAsync(c->d_birth_hist, 0, sizeof(uint32_t) * (size_t)steps_need, c->stream));
-    HIP_CHECK(c, hipMemsetAsync(c->d_n_live, 0, sizeof(uint32_t) * (size_t)steps_need, c->stream));
-    HIP_CHECK(c, hipMemsetAsync(c->d_queue, 0, sizeof(uint32_t) * (size_t)steps_need, c->stream));
-    HIP_CHECK(c, hipEventRecord(c->ev[0], c->stream));
+    plan.n = n;
+    plan.N = N;
+    plan.R = R;
+    plan.S = S;
+    plan.blocks_needed = blocks_needed;
+    plan.grid = grid;
+    plan.steps_need = steps_need;
+    plan.WS = WS;
+    plan.W = W;
+    plan.global_tally = global_tally;
+    plan.windows = windows;
+    return PSIM_OK;
+}
+
+int zero_fill(psim_gpu_ctx* c, void* p, size_t bytes) {
+    const int64_t words = (int64_t)(bytes / 4);
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (words + BLOCK - 1) / BLOCK));
+    hipLaunchKernelGGL(zero_words_kernel, dim3(blocks), dim3(BLOCK), 0, c->stream, static_cast<uint32_t*>(p), words);
+    HIP_CHECK(c, hipGetLastError());
+    return PSIM_OK;
+}
+
+// Every launch of one run, on c->stream.  `eager`: also times each transport window with events
+// (and prints a progress line per window when PSIM_PROGRESS is set); otherwise the sequence is
+// free of host synchronisation, so it can be captured into a graph.
+int enqueue_run(psim_gpu_ctx* c, const RunPlan& plan, bool eager) {
+    const psim_run_params& P = c->params;
+    const int64_t n = plan.n, N = plan.N, R = plan.R, WS = plan.WS, blocks_needed = plan.blocks_needed;
+    const int S = plan.S, grid = plan.grid;
+    const int32_t W = plan.W;
+    const bool global_tally = plan.global_tally;
+    const auto& windows = plan.windows;
+    const int64_t n_windows = (int64_t)windows.size();
+    const Fate fate = c->fate_enabled ? c->fate : Fate{ nullptr, nullptr, nullptr };
+    int rc;
+    if ((rc = zero_fill(c, c->d_counts, sizeof(int64_t) * (size_t)std::max<int64_t>(1, S * R))) != 0) { return rc; }
+    if ((rc = zero_fill(c, c->d_flux, sizeof(double) * (size_t)std::max<int64_t>(1, 2 * S * R))) != 0) { return rc; }
+    if ((rc = zero_fill(c, c->d_counters, sizeof(unsigned long long) * PSIM_NUM_COUNTERS)) != 0) { return rc; }
+    if ((rc = zero_fill(c, c->d_birth_hist, sizeof(uint32_t) * (size_t)plan.steps_need)) != 0) { return rc; }
+    if ((rc = zero_fill(c, c->d_n_live, sizeof(uint32_t) * (size_t)plan.steps_need)) != 0) { return rc; }
+    if ((rc = zero_fill(c, c->d_queue, sizeof(uint32_t) * (size_t)plan.steps_need)) != 0) { return rc; }
     if (n > 0) {
         hipLaunchKernelGGL(emit_kernel, dim3((unsigned)blocks_needed), dim3(BLOCK), 0, c->stream, c->m, c->pool, n,
-            seed, c->d_counters, fate, c->d_birth_key, c->d_birth_hist);
+            c->d_seed, c->d_counters, fate, c->d_birth_key, c->d_birth_hist);
         HIP_CHECK(c, hipGetLastError());
         hipLaunchKernelGGL(birth_scan_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_birth_hist, N, c->d_born_offset,
             c->d_cursor);
@@ -1025,8 +1092,6 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
             c->d_birth_key, n, c->d_cursor, c->d_born_order);
         HIP_CHECK(c, hipGetLastError());
     }
-    c->step_ms = 0.;
-    c->step_launches = 0;
     const bool progress = std::getenv("PSIM_PROGRESS") != nullptr;  // one stderr line per window
     for (int32_t w = 0; w < (int32_t)n_windows && n > 0; ++w) {
         const int64_t k0 = windows[(size_t)w].first, k1 = windows[(size_t)w].second;
@@ -1038,11 +1103,16 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
         const bool rec = (k1 >= P.step_adjustment) && (k0 + 1 - P.step_adjustment < R) && (k1 - k0 <= W);
         Tally t{ c->d_pcnt, c->d_pfx, c->d_pfy, c->d_counts, c->d_flux, rec ? 1 : 0, global_tally ? 1 : 0, W };
         const size_t lds = (rec && !global_tally) ? (size_t)WS * 20 : 0;
-        HIP_CHECK(c, hipEventRecord(c->ev[1], c->stream));
-        hipLaunchKernelGGL(transport_kernel, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool, lists, (int32_t)k0,
-            (int32_t)k1, seed, t, c->d_counters, fate);
+        if (eager) { HIP_CHECK(c, hipEventRecord(c->ev[1], c->stream)); }
+        if (global_tally) {
+            hipLaunchKernelGGL(transport_kernel<true>, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool, lists,
+                (int32_t)k0, (int32_t)k1, c->d_seed, t, c->d_counters, fate);
+        } else {
+            hipLaunchKernelGGL(transport_kernel<false>, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool, lists,
+                (int32_t)k0, (int32_t)k1, c->d_seed, t, c->d_counters, fate);
+        }
         HIP_CHECK(c, hipGetLastError());
-        HIP_CHECK(c, hipEventRecord(c->ev[2], c->stream));
+        if (eager) { HIP_CHECK(c, hipEventRecord(c->ev[2], c->stream)); }
         if (rec && !global_tally) {
             const int rblocks = (int)((WS + BLOCK - 1) / BLOCK);
             hipLaunchKernelGGL(reduce_kernel, dim3(rblocks), dim3(BLOCK), 0, c->stream, c->d_pcnt, c->d_pfx, c->d_pfy,
@@ -1055,7 +1125,7 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
             const int n_keys = ((c->n_cells - 1) >> shift) + 1;
             const uint32_t* n_ptr = c->d_n_live + w + 1;
             const int sgrid = std::max(1, c->n_cu * 2);
-            HIP_CHECK(c, hipMemsetAsync(c->d_sort_keys, 0, sizeof(uint32_t) * (size_t)n_keys, c->stream));
+            if ((rc = zero_fill(c, c->d_sort_keys, sizeof(uint32_t) * (size_t)n_keys)) != 0) { return rc; }
             hipLaunchKernelGGL(cell_hist_kernel, dim3(sgrid), dim3(SORT_BLOCK), 0, c->stream, c->live[1].cell, n_ptr,
                 shift, n_keys, c->d_sort_keys);
             HIP_CHECK(c, hipGetLastError());
@@ -1066,6 +1136,7 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
                 n_ptr, shift, n_keys, c->d_sort_keys);
             HIP_CHECK(c, hipGetLastError());
         }
+        if (!eager) { continue; }
         HIP_CHECK(c, hipEventSynchronize(c->ev[2]));
         float ms = 0.f;
         HIP_CHECK(c, hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
@@ -1077,11 +1148,70 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
             std::fflush(stderr);
         }
     }
+    return PSIM_OK;
+}
+
+int set_seed(psim_gpu_ctx* c, uint64_t seed) {
+    *c->h_seed = seed;
+    HIP_CHECK(c, hipMemcpyAsync(c->d_seed, c->h_seed, sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+    return PSIM_OK;
+}
+
+int finish_run(psim_gpu_ctx* c) {
     HIP_CHECK(c, hipEventRecord(c->ev[3], c->stream));
     HIP_CHECK(c, hipEventSynchronize(c->ev[3]));
     float total = 0.f;
     HIP_CHECK(c, hipEventElapsedTime(&total, c->ev[0], c->ev[3]));
     c->run_ms = total;
+    return PSIM_OK;
+}
+
+int run_impl(psim_gpu_ctx* c, uint64_t seed) {
+    RunPlan plan;
+    int rc;
+    if ((rc = prepare_run(c, plan)) != 0) { return rc; }
+    if ((rc = set_seed(c, seed)) != 0) { return rc; }
+    c->step_ms = 0.;
+    c->step_launches = 0;
+    HIP_CHECK(c, hipEventRecord(c->ev[0], c->stream));
+    if ((rc = enqueue_run(c, plan, true)) != 0) { return rc; }
+    return finish_run(c);
+}
+
+// The same run replayed from a graph captured on first use (and again whenever c->gen moves).
+int run_graph_impl(psim_gpu_ctx* c, uint64_t seed) {
+    RunPlan plan;
+    int rc;
+    if ((rc = prepare_run(c, plan)) != 0) { return rc; }
+    if (!c->graph_exec || c->graph_gen != c->gen) {
+        if (c->graph_exec) {
+            (void)hipGraphExecDestroy(c->graph_exec);
+            c->graph_exec = nullptr;
+        }
+        HIP_CHECK(c, hipStreamSynchronize(c->stream));
+        HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_run(c, plan, false);
+        hipGraph_t graph = nullptr;
+        const hipError_t end = hipStreamEndCapture(c->stream, &graph);
+        if (rc != 0) {
+            if (graph) { (void)hipGraphDestroy(graph); }
+            return rc;
+        }
+        if (end != hipSuccess) { return fail(c, PSIM_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(end)); }
+        const hipError_t inst = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (inst != hipSuccess) {
+            c->graph_exec = nullptr;
+            return fail(c, PSIM_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(inst));
+        }
+        c->graph_gen = c->gen;
+    }
+    if ((rc = set_seed(c, seed)) != 0) { return rc; }
+    HIP_CHECK(c, hipEventRecord(c->ev[0], c->stream));
+    HIP_CHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    if ((rc = finish_run(c)) != 0) { return rc; }
+    c->step_ms = c->run_ms;
+    c->step_launches = (int64_t)plan.windows.size();
     return PSIM_OK;
 }
 
@@ -1118,7 +1248,10 @@ int psim_gpu_create(psim_gpu_ctx** out, int device) {
         }
     }
     c->d_counters = dalloc<unsigned long long>(PSIM_NUM_COUNTERS);
-    if (!c->d_counters) {
+    c->d_seed = dalloc<uint64_t>(1);
+    if (!c->d_counters || !c->d_seed
+        || hipHostMalloc(reinterpret_cast<void**>(&c->h_seed), sizeof(uint64_t), hipHostMallocDefault) != hipSuccess) {
+        c->h_seed = nullptr;
         psim_gpu_destroy(c);
         return PSIM_E_DEVICE;
     }
@@ -1139,6 +1272,9 @@ void psim_gpu_destroy(psim_gpu_ctx* c) {
     if (c->d_counts) { (void)hipFree(c->d_counts); }
     if (c->d_flux) { (void)hipFree(c->d_flux); }
     if (c->d_counters) { (void)hipFree(c->d_counters); }
+    if (c->d_seed) { (void)hipFree(c->d_seed); }
+    if (c->h_seed) { (void)hipHostFree(c->h_seed); }
+    if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); }
     for (auto& e : c->ev) {
         if (e) { (void)hipEventDestroy(e); }
     }
@@ -1284,6 +1420,7 @@ int psim_gpu_upload(psim_gpu_ctx* c, const psim_geom* g, const psim_tables* t, c
     c->n_sensors = g->n_sensors;
     c->n_cells = g->n_cells;
     c->uploaded = true;
+    c->gen++;
     return PSIM_OK;
 }
 
@@ -1318,6 +1455,7 @@ int psim_gpu_run(psim_gpu_ctx* c, uint64_t seed, int64_t* counts, double* flux, 
 
 int psim_gpu_set_fate(psim_gpu_ctx* c, int enable) {
     if (!c) { return PSIM_E_INVALID; }
+    if (c->fate_enabled != (enable != 0)) { c->gen++; }
     c->fate_enabled = enable != 0;
     return PSIM_OK;
 }
@@ -1344,7 +1482,59 @@ int psim_gpu_last_timing(const psim_gpu_ctx* c, double* step_ms, int64_t* launch
 
 int psim_gpu_set_window(psim_gpu_ctx* c, int32_t steps) {
     if (!c || steps < 0) { return PSIM_E_INVALID; }
+    if (c->window != steps) { c->gen++; }
     c->window = steps;
+    return PSIM_OK;
+}
+
+int psim_gpu_set_grid(psim_gpu_ctx* c, int32_t blocks_per_cu) {
+    if (!c || blocks_per_cu < 1 || blocks_per_cu > 64) { return PSIM_E_INVALID; }
+    if (c->blocks_per_cu != blocks_per_cu) { c->gen++; }
+    c->blocks_per_cu = blocks_per_cu;
+    return PSIM_OK;
+}
+
+int psim_gpu_run_graph(psim_gpu_ctx* c, uint64_t seed, int64_t* counts, double* flux, int64_t* counters) {
+    if (!c) { return PSIM_E_INVALID; }
+    const int rc = run_graph_impl(c, seed);
+    if (rc != PSIM_OK) { return rc; }
+    return psim_gpu_fetch(c, counts, flux, counters);
+}
+
+int psim_gpu_autotune(psim_gpu_ctx* c, uint64_t seed, int32_t* window, int32_t* blocks_per_cu, double* best_ms) {
+    if (!c) { return PSIM_E_INVALID; }
+    // Launch shapes that leave every result unchanged: the length of the untallied windows
+    // (0 = one launch; shorter ones re-sort the survivors by cell more often) and the number of
+    // transport blocks per CU.  Each candidate runs the uploaded model once; the fastest is kept.
+    const int32_t windows[] = { 0, 1024, 256 };
+    const int32_t grids[] = { 8, 3 };
+    const int32_t w0 = c->window, g0 = c->blocks_per_cu;
+    double best = -1.;
+    int32_t bw = w0, bg = g0;
+    for (int32_t g : grids) {
+        for (int32_t w : windows) {
+            c->window = w;
+            c->blocks_per_cu = g;
+            c->gen++;
+            const int rc = run_impl(c, seed);
+            if (rc != PSIM_OK) {
+                c->window = w0;
+                c->blocks_per_cu = g0;
+                return rc;
+            }
+            if (best < 0. || c->run_ms < best) {
+                best = c->run_ms;
+                bw = w;
+                bg = g;
+            }
+        }
+    }
+    c->window = bw;
+    c->blocks_per_cu = bg;
+    c->gen++;
+    if (window) { *window = bw; }
+    if (blocks_per_cu) { *blocks_per_cu = bg; }
+    if (best_ms) { *best_ms = best; }
     return PSIM_OK;
 }
 
