@@ -1,0 +1,54 @@
+/*
+ * psim_surrogate.h -- C ABI of the steady-state surrogate used for warm starts.
+ *
+ * No counterpart in the reference: psim starts every sensor at its configured initial
+ * temperature (SensorController::t_steady_ -> getInitTemp,
+ * psim/include/psim/sensorController.h:63-67).  The surrogate is an MLP
+ * n_in -> hidden (x layers, ReLU) -> 1 whose weights and activations are bfloat16 and whose
+ * products accumulate in FP32 on the MFMA units; Adam keeps FP32 master weights.  It proposes
+ * initial sensor temperatures only; results always come from the Monte Carlo.
+ *
+ * Parameters are one flat FP32 vector: for each layer l (inputs dims[l], outputs dims[l + 1],
+ * dims = n_in, hidden x layers, 16), the weights [dims[l]][dims[l + 1]] row-major, then the
+ * biases [dims[l + 1]].  Only column 0 of the last layer is used; its other 15 columns stay 0.
+ * Every function returns 0 or a negative PSIM_E_* code (psim_gpu.h).
+ */
+#ifndef PSIM_SURROGATE_H
+#define PSIM_SURROGATE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct psim_mlp psim_mlp;
+
+int psim_mlp_create(psim_mlp** mlp, int device, int32_t n_in, int32_t hidden, int32_t layers);
+void psim_mlp_destroy(psim_mlp* mlp);
+const char* psim_mlp_last_error(const psim_mlp* mlp);
+int64_t psim_mlp_num_params(const psim_mlp* mlp);
+
+/* Sets the FP32 master parameters (and their bf16 copy); resets the optimiser state. */
+int psim_mlp_set_params(psim_mlp* mlp, const float* params);
+int psim_mlp_get_params(psim_mlp* mlp, float* params);
+
+/* y[r] = network(x[r][0 .. n_in)) for r < rows. */
+int psim_mlp_forward(psim_mlp* mlp, const float* x, int32_t rows, float* y);
+
+/* Mean squared error against t and its gradient with respect to every parameter (FP32). */
+int psim_mlp_gradients(psim_mlp* mlp, const float* x, const float* t, int32_t rows, float* grad, float* loss);
+
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the mean squared error of one batch. */
+int psim_mlp_train_step(psim_mlp* mlp, const float* x, const float* t, int32_t rows, float lr, float* loss);
+
+/* C[M][N] = op(A)[M][K] x op(B)[K][N] on the MFMA GEMM used by the network, with the operands
+ * rounded to bf16 (A stored [K][M] when ta, B stored [N][K] when tb): a check of the kernel. */
+int psim_mlp_gemm_check(int device, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const float* A,
+    const float* B, float* C);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSIM_SURROGATE_H */

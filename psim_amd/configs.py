@@ -163,57 +163,71 @@ def with_settings(model: dict, **settings) -> dict:
     return m
 
 
-def batch_geometry(rng, num_measurements: int = 2000, sim_time: float = 10.0) -> dict:
-    """One structure of the batched-geometry workload, drawn from `rng` (random.Random):
+def batch_geometry_params(rng) -> dict:
+    """Draws the parameters of one batched-geometry structure from `rng` (random.Random):
       - a straight bar (length 200-5000 nm, width 20-500 nm), or a kinked one: two arms of half
         that length whose second arm turns by 15-90 degrees about the inner corner, joined by a
         wedge-shaped sensor; every shared edge is shared whole;
       - 4-200 sensors along its length (the wedge is one of them), each split into two triangles;
       - emitting walls across both ends at independent temperatures U(250, 350) K;
-      - one specularity U(0, 1) for every cell, a phonon budget log-uniform in [1e4, 1e6];
-      - deviational steady state about the mean wall temperature (t_eq = t_init = mean)."""
+      - one specularity U(0, 1) for every cell, a phonon budget log-uniform in [1e4, 1e6]."""
     import math
 
-    kinked = rng.random() < 0.5
-    n_sensors = rng.randint(4, 200)
-    length = rng.uniform(200.0, 5000.0)
-    width = rng.uniform(20.0, 500.0)
-    t_hot, t_cold = rng.uniform(250.0, 350.0), rng.uniform(250.0, 350.0)
-    spec = rng.uniform(0.0, 1.0)
-    phonons = int(round(10.0 ** rng.uniform(4.0, 6.0)))
-    t_eq = 0.5 * (t_hot + t_cold)
-    b = ModelBuilder(num_measurements, sim_time, phonons, t_eq, sim_type=0)
+    p = {"kinked": rng.random() < 0.5, "n_sensors": rng.randint(4, 200), "length": rng.uniform(200.0, 5000.0),
+         "width": rng.uniform(20.0, 500.0), "t_hot": rng.uniform(250.0, 350.0), "t_cold": rng.uniform(250.0, 350.0),
+         "spec": rng.uniform(0.0, 1.0), "phonons": int(round(10.0 ** rng.uniform(4.0, 6.0)))}
+    p["theta"] = math.radians(rng.uniform(15.0, 90.0)) if p["kinked"] else 0.0
+    return p
+
+
+def geometry_from_params(p: dict, num_measurements: int = 2000, sim_time: float = 10.0, sim_type: int = 0,
+                         step_interval: int = 0, t_init=None) -> dict:
+    """The model of one batched-geometry structure: deviational about the mean wall temperature
+    (t_eq = mean); every sensor starts at t_init[i] (default t_eq).  Sensors are numbered along the
+    structure from the hot wall; a kinked structure has (n - 1) // 2 sensors in its first arm,
+    then the wedge, then the second arm."""
+    import math
+
+    n_sensors, length, width, spec = p["n_sensors"], p["length"], p["width"], p["spec"]
+    t_eq = 0.5 * (p["t_hot"] + p["t_cold"])
+    b = ModelBuilder(num_measurements, sim_time, p["phonons"], t_eq, sim_type=sim_type, step_interval=step_interval)
+    inits = [t_eq] * n_sensors if t_init is None else [float(v) for v in t_init]
 
     def quad(a, bb, c, d, sid):  # a-b-c-d counter-clockwise, split along a-c
         b.add_triangle(a, bb, c, sid, spec)
         b.add_triangle(a, c, d, sid, spec)
 
-    if not kinked:
+    if not p["kinked"]:
         xs = [length * i / n_sensors for i in range(n_sensors + 1)]
         for i in range(n_sensors):
-            quad((xs[i], 0.0), (xs[i + 1], 0.0), (xs[i + 1], width), (xs[i], width), b.add_sensor(t_eq))
-        b.add_surface((0.0, 0.0), (0.0, width), t_hot)
-        b.add_surface((length, 0.0), (length, width), t_cold)
+            quad((xs[i], 0.0), (xs[i + 1], 0.0), (xs[i + 1], width), (xs[i], width), b.add_sensor(inits[i]))
+        b.add_surface((0.0, 0.0), (0.0, width), p["t_hot"])
+        b.add_surface((length, 0.0), (length, width), p["t_cold"])
         return b.to_dict()
-    theta = math.radians(rng.uniform(15.0, 90.0))
+    theta = p["theta"]
     n1 = (n_sensors - 1) // 2
     n2 = n_sensors - 1 - n1
     l1 = l2 = 0.5 * length
     xs = [l1 * i / n1 for i in range(n1 + 1)]
     for i in range(n1):
-        quad((xs[i], 0.0), (xs[i + 1], 0.0), (xs[i + 1], width), (xs[i], width), b.add_sensor(t_eq))
+        quad((xs[i], 0.0), (xs[i + 1], 0.0), (xs[i + 1], width), (xs[i], width), b.add_sensor(inits[i]))
     q = (l1, width)  # inner corner; the second arm starts on edge a2-q, the first arm's end edge turned by theta
     a = (l1, 0.0)
     a2 = (q[0] + width * math.sin(theta), q[1] - width * math.cos(theta))
-    b.add_triangle(a, a2, q, b.add_sensor(t_eq), spec)  # the wedge
+    b.add_triangle(a, a2, q, b.add_sensor(inits[n1]), spec)  # the wedge
     d = (math.cos(theta), math.sin(theta))
     lo = [(a2[0] + l2 * d[0] * j / n2, a2[1] + l2 * d[1] * j / n2) for j in range(n2 + 1)]
     hi = [(q[0] + l2 * d[0] * j / n2, q[1] + l2 * d[1] * j / n2) for j in range(n2 + 1)]
     for j in range(n2):
-        quad(lo[j], lo[j + 1], hi[j + 1], hi[j], b.add_sensor(t_eq))
-    b.add_surface((0.0, 0.0), (0.0, width), t_hot)
-    b.add_surface(lo[n2], hi[n2], t_cold)
+        quad(lo[j], lo[j + 1], hi[j + 1], hi[j], b.add_sensor(inits[n1 + 1 + j]))
+    b.add_surface((0.0, 0.0), (0.0, width), p["t_hot"])
+    b.add_surface(lo[n2], hi[n2], p["t_cold"])
     return b.to_dict()
+
+
+def batch_geometry(rng, num_measurements: int = 2000, sim_time: float = 10.0) -> dict:
+    """One structure of the batched-geometry workload (batch_geometry_params, geometry_from_params)."""
+    return geometry_from_params(batch_geometry_params(rng), num_measurements, sim_time)
 
 
 def batch_geometries(seed: int = 0, count: int = 256, num_measurements: int = 2000, sim_time: float = 10.0) -> list:

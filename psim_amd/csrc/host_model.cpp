@@ -684,13 +684,17 @@ void Model::addCell(const Triangle& triangle, std::size_t sensor_id, double spec
         if (!triEq(a, b)) {
             // Cell::validate, cell.cpp:21-25, with IntersectError / OverlapError (cell.cpp:150-164);
             // IntersectError(existing = this cell, incoming = the other one) prints them that way.
-            const std::string ta = fmtTriangle(a.p1, a.p2, a.p3), tb = fmtTriangle(b.p1, b.p2, b.p3);
-            if (triIntersects(a, b)) { throw std::runtime_error("Incoming " + tb + "\nintersects\nExisting" + ta + '\n'); }
+            // The messages are formatted only when thrown: this loop visits every pair of cells.
+            const auto ta = [&] { return fmtTriangle(a.p1, a.p2, a.p3); };
+            const auto tb = [&] { return fmtTriangle(b.p1, b.p2, b.p3); };
+            if (triIntersects(a, b)) {
+                throw std::runtime_error("Incoming " + tb() + "\nintersects\nExisting" + ta() + '\n');
+            }
             if (triContainsPoint(a, b.p1) || triContainsPoint(a, b.p2) || triContainsPoint(a, b.p3)) {
-                throw std::runtime_error(tb + "\nis contained within\n" + ta + '\n');
+                throw std::runtime_error(tb() + "\nis contained within\n" + ta() + '\n');
             }
             if (triContainsPoint(b, a.p1) || triContainsPoint(b, a.p2) || triContainsPoint(b, a.p3)) {
-                throw std::runtime_error(ta + "\nis contained within\n" + tb + '\n');
+                throw std::runtime_error(ta() + "\nis contained within\n" + tb() + '\n');
             }
             // Cell::findTransitionSurface, cell.cpp:81-98
             const std::array<Line, 3> l_inc = { cells_[inc].boundaries()[0].line, cells_[inc].boundaries()[1].line,

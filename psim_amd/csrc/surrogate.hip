@@ -1,0 +1,511 @@
+// Steady-state surrogate for warm starts: an MLP n_in -> hidden x layers -> 1 trained and run on the
+// GPU by the kernels below.  Weights and activations are bfloat16, every product accumulates in
+// FP32 on the MFMA units (v_mfma_f32_16x16x32_bf16), and the optimiser (Adam) keeps FP32 master
+// weights.  The reference has no counterpart: it starts every sensor at its configured initial
+// temperature (SensorController::t_steady_ -> getInitTemp, psim/include/psim/sensorController.h:63-67);
+// the surrogate only proposes those initial temperatures, the Monte Carlo result stays the answer.
+//
+// Layout in HBM: activations row-major [batch][features] (bf16), weights [in][out] (bf16 copy of
+// the FP32 master), pre-activations [batch][out] (FP32, kept for the ReLU backward).  The output
+// layer is padded to PAD_OUT columns with weights that stay zero, so every product is a GEMM on
+// 16 x 16 MFMA tiles.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "psim_gpu.h"
+#include "psim_surrogate.h"
+
+namespace {
+
+constexpr int TILE = 64;  // output tile of a GEMM block
+constexpr int KSTEP = 32;  // K of one MFMA
+constexpr int GEMM_THREADS = 256;  // 4 waves; wave w computes rows [16 w, 16 w + 16) of the tile
+constexpr int PAD_OUT = 16;  // output-layer columns (1 used)
+constexpr int EW_BLOCK = 256;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// float -> bfloat16 bits, round to nearest even (NaN kept quiet).
+__host__ __device__ inline uint16_t f32_to_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) { return (uint16_t)((u >> 16) | 0x40u); }
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__host__ __device__ inline float bf16_to_f32(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// C[M][N] = op(A)[M][K] x op(B)[K][N] in FP32.  Stored operands (bf16 bits):
+//   A: TA ? [K][M] : [M][K];  B: TB ? [N][K] : [K][N].
+// Both operands are staged through LDS K-contiguous (As[m][k], Bs[n][k]), so a lane's eight K
+// values of its MFMA fragment are one 16-byte read.  Out-of-range elements load as zero.
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(GEMM_THREADS) gemm_bf16_kernel(const uint16_t* __restrict__ A,
+    const uint16_t* __restrict__ B, float* __restrict__ C, int M, int N, int K) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[TILE][KSTEP + 8];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[TILE][KSTEP + 8];
+    const int tid = threadIdx.x;
+    const int wave = tid / 64, lane = tid % 64;
+    const int m0 = blockIdx.y * TILE, n0 = blockIdx.x * TILE;
+    f32x4 acc[4];
+    for (int c = 0; c < 4; ++c) { acc[c] = f32x4{ 0.f, 0.f, 0.f, 0.f }; }
+    for (int k0 = 0; k0 < K; k0 += KSTEP) {
+        // each thread stages 8 elements of each operand tile (64 x 32)
+        for (int e = tid; e < TILE * KSTEP; e += GEMM_THREADS) {
+            const int r = e / KSTEP, kk = e % KSTEP;  // tile row (m or n), k within the step
+            const int m = m0 + r, n = n0 + r, k = k0 + kk;
+            uint16_t a = 0, b = 0;
+            if (m < M && k < K) { a = TA ? A[(int64_t)k * M + m] : A[(int64_t)m * K + k]; }
+            if (n < N && k < K) { b = TB ? B[(int64_t)n * K + k] : B[(int64_t)k * N + n]; }
+            As[r][kk] = a;
+            Bs[r][kk] = b;
+        }
+        __syncthreads();
+        const int kq = (lane / 16) * 8;
+        bf16x8 fa;
+        std::memcpy(&fa, &As[wave * 16 + lane % 16][kq], 16);
+        for (int c = 0; c < 4; ++c) {
+            bf16x8 fb;
+            std::memcpy(&fb, &Bs[c * 16 + lane % 16][kq], 16);
+            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc[c], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D fragment: lane holds column lane % 16, rows 4 (lane / 16) + i
+    for (int c = 0; c < 4; ++c) {
+        const int n = n0 + c * 16 + lane % 16;
+        for (int i = 0; i < 4; ++i) {
+            const int m = m0 + wave * 16 + (lane / 16) * 4 + i;
+            if (m < M && n < N) { C[(int64_t)m * N + n] = acc[c][i]; }
+        }
+    }
+}
+
+// z += bias (per column); a = relu ? max(z, 0) : z, rounded to bf16.
+__global__ void bias_act_kernel(float* __restrict__ z, const float* __restrict__ bias, uint16_t* __restrict__ a,
+    int64_t rows, int cols, int relu) {
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= rows * cols) { return; }
+    const float v = z[i] + bias[i % cols];
+    z[i] = v;
+    a[i] = f32_to_bf16(relu ? fmaxf(v, 0.f) : v);
+}
+
+// Output-layer gradient of the mean squared error on column 0: dz[r][0] = 2 (y - t) / rows, other
+// columns 0 (bf16), and the loss terms (y - t)^2 / rows per row.
+__global__ void loss_grad_kernel(const float* __restrict__ z, const float* __restrict__ t, uint16_t* __restrict__ dz,
+    float* __restrict__ loss_terms, int rows, int cols) {
+    const int r = blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (r >= rows) { return; }
+    const float d = z[(int64_t)r * cols] - t[r];
+    loss_terms[r] = d * d / (float)rows;
+    for (int c = 0; c < cols; ++c) { dz[(int64_t)r * cols + c] = f32_to_bf16(c == 0 ? 2.f * d / (float)rows : 0.f); }
+}
+
+// dz = da * (z > 0), rounded to bf16 (da FP32 from the GEMM).
+__global__ void relu_back_kernel(const float* __restrict__ da, const float* __restrict__ z, uint16_t* __restrict__ dz,
+    int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) { return; }
+    dz[i] = f32_to_bf16(z[i] > 0.f ? da[i] : 0.f);
+}
+
+// db[c] = sum over rows of dz[r][c] (one block per column, FP32).
+__global__ void bias_grad_kernel(const uint16_t* __restrict__ dz, float* __restrict__ db, int rows, int cols) {
+    __shared__ float part[EW_BLOCK];
+    const int c = blockIdx.x;
+    float s = 0.f;
+    for (int r = threadIdx.x; r < rows; r += EW_BLOCK) { s += bf16_to_f32(dz[(int64_t)r * cols + c]); }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = EW_BLOCK / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) { part[threadIdx.x] += part[threadIdx.x + off]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { db[c] = part[0]; }
+}
+
+// Adam on the FP32 master parameters, then their bf16 copy (weights only; masked entries stay 0).
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
+    float* __restrict__ m2, const float* __restrict__ mask, uint16_t* __restrict__ pb, int64_t n, float lr, float b1,
+    float b2, float eps, float c1, float c2) {
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) { return; }
+    const float gi = g[i] * (mask ? mask[i] : 1.f);
+    const float a = b1 * m1[i] + (1.f - b1) * gi;
+    const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    const float v = p[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+    p[i] = v;
+    if (pb) { pb[i] = f32_to_bf16(v); }
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ x, uint16_t* __restrict__ y, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i < n) { y[i] = f32_to_bf16(x[i]); }
+}
+
+unsigned ew_blocks(int64_t n) { return (unsigned)((n + EW_BLOCK - 1) / EW_BLOCK); }
+
+}  // namespace
+
+struct psim_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int n_in = 0, hidden = 0, layers = 0;  // layers = hidden layers; weight matrices = layers + 1
+    std::vector<int> dims;  // n_in, hidden..., PAD_OUT
+    std::vector<int64_t> w_off, b_off;  // offsets in the flat parameter vector
+    int64_t n_params = 0;
+    float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr, *mask = nullptr;
+    uint16_t* pb = nullptr;  // bf16 copy (weights used by the GEMMs; biases unused)
+    int64_t step = 0;
+    // batch buffers
+    int cap = 0;
+    std::vector<float*> z;  // pre-activations per layer [cap][dims[l + 1]]
+    std::vector<uint16_t*> a;  // activations: a[0] = input, a[l + 1] = output of layer l
+    float* da = nullptr;  // FP32 gradient from a GEMM [cap][hidden]
+    uint16_t* dz = nullptr;  // bf16 gradient of the current layer's pre-activation
+    uint16_t* dz2 = nullptr;
+    float* xin = nullptr;
+    float* t = nullptr;
+    float* loss = nullptr;
+};
+
+namespace {
+
+int mlp_fail(psim_mlp* h, int code, const std::string& msg) {
+    if (h) { h->err = msg; }
+    return code;
+}
+
+#define MLP_CHECK(h, expr)                                                                                   \
+    do {                                                                                                     \
+        const hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) { return mlp_fail((h), PSIM_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); } \
+    } while (0)
+
+template <typename T>
+T* mlp_alloc(size_t n) {
+    T* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) { return nullptr; }
+    return p;
+}
+
+void free_batch(psim_mlp* h) {
+    for (float* q : h->z) { (void)hipFree(q); }
+    for (uint16_t* q : h->a) { (void)hipFree(q); }
+    h->z.clear();
+    h->a.clear();
+    void* rest[] = { h->da, h->dz, h->dz2, h->xin, h->t, h->loss };
+    for (void* q : rest) {
+        if (q) { (void)hipFree(q); }
+    }
+    h->da = nullptr;
+    h->dz = h->dz2 = nullptr;
+    h->xin = h->t = h->loss = nullptr;
+    h->cap = 0;
+}
+
+int ensure_batch(psim_mlp* h, int rows) {
+    if (rows <= h->cap) { return PSIM_OK; }
+    free_batch(h);
+    const int cap = ((rows + TILE - 1) / TILE) * TILE;
+    const int L = (int)h->dims.size() - 1;
+    for (int l = 0; l < L; ++l) { h->z.push_back(mlp_alloc<float>((size_t)cap * h->dims[l + 1])); }
+    for (int l = 0; l <= L; ++l) { h->a.push_back(mlp_alloc<uint16_t>((size_t)cap * h->dims[l])); }
+    int widest = 0;
+    for (int d : h->dims) { widest = std::max(widest, d); }
+    h->da = mlp_alloc<float>((size_t)cap * widest);
+    h->dz = mlp_alloc<uint16_t>((size_t)cap * widest);
+    h->dz2 = mlp_alloc<uint16_t>((size_t)cap * widest);
+    h->xin = mlp_alloc<float>((size_t)cap * h->n_in);
+    h->t = mlp_alloc<float>((size_t)cap);
+    h->loss = mlp_alloc<float>((size_t)cap);
+    bool ok = h->da && h->dz && h->dz2 && h->xin && h->t && h->loss;
+    for (float* q : h->z) { ok = ok && q; }
+    for (uint16_t* q : h->a) { ok = ok && q; }
+    if (!ok) {
+        free_batch(h);
+        return mlp_fail(h, PSIM_E_DEVICE, "hipMalloc failed for the surrogate batch buffers");
+    }
+    h->cap = cap;
+    return PSIM_OK;
+}
+
+template <bool TA, bool TB>
+int gemm(psim_mlp* h, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K) {
+    const dim3 grid((unsigned)((N + TILE - 1) / TILE), (unsigned)((M + TILE - 1) / TILE));
+    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB>), grid, dim3(GEMM_THREADS), 0, h->stream, A, B, C, M, N, K);
+    MLP_CHECK(h, hipGetLastError());
+    return PSIM_OK;
+}
+
+// Forward pass over `rows` inputs already in h->xin; leaves z and a of every layer.
+int forward(psim_mlp* h, int rows) {
+    int rc;
+    const int L = (int)h->dims.size() - 1;
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(ew_blocks((int64_t)rows * h->n_in)), dim3(EW_BLOCK), 0, h->stream, h->xin,
+        h->a[0], (int64_t)rows * h->n_in);
+    MLP_CHECK(h, hipGetLastError());
+    for (int l = 0; l < L; ++l) {
+        const int K = h->dims[l], N = h->dims[l + 1];
+        if ((rc = gemm<false, false>(h, h->a[l], h->pb + h->w_off[l], h->z[l], rows, N, K)) != 0) { return rc; }
+        hipLaunchKernelGGL(bias_act_kernel, dim3(ew_blocks((int64_t)rows * N)), dim3(EW_BLOCK), 0, h->stream, h->z[l],
+            h->p + h->b_off[l], h->a[l + 1], (int64_t)rows, N, l + 1 < L ? 1 : 0);
+        MLP_CHECK(h, hipGetLastError());
+    }
+    return PSIM_OK;
+}
+
+// Loss and gradients of every parameter (into h->g) for targets in h->t.
+int backward(psim_mlp* h, int rows) {
+    int rc;
+    const int L = (int)h->dims.size() - 1;
+    hipLaunchKernelGGL(loss_grad_kernel, dim3(ew_blocks(rows)), dim3(EW_BLOCK), 0, h->stream, h->z[L - 1], h->t, h->dz,
+        h->loss, rows, h->dims[L]);
+    MLP_CHECK(h, hipGetLastError());
+    uint16_t* dz = h->dz;
+    uint16_t* spare = h->dz2;
+    for (int l = L - 1; l >= 0; --l) {
+        const int K = h->dims[l], N = h->dims[l + 1];
+        // dW[K][N] = a_l^T [K][rows] x dz [rows][N]; db[N] = column sums of dz
+        if ((rc = gemm<true, false>(h, h->a[l], dz, h->g + h->w_off[l], K, N, rows)) != 0) { return rc; }
+        hipLaunchKernelGGL(bias_grad_kernel, dim3((unsigned)N), dim3(EW_BLOCK), 0, h->stream, dz, h->g + h->b_off[l],
+            rows, N);
+        MLP_CHECK(h, hipGetLastError());
+        if (l == 0) { break; }
+        // da[rows][K] = dz [rows][N] x W^T [N][K] (W stored [K][N]), then through the ReLU of layer l - 1
+        if ((rc = gemm<false, true>(h, dz, h->pb + h->w_off[l], h->da, rows, K, N)) != 0) { return rc; }
+        hipLaunchKernelGGL(relu_back_kernel, dim3(ew_blocks((int64_t)rows * K)), dim3(EW_BLOCK), 0, h->stream, h->da,
+            h->z[l - 1], spare, (int64_t)rows * K);
+        MLP_CHECK(h, hipGetLastError());
+        std::swap(dz, spare);
+    }
+    return PSIM_OK;
+}
+
+int load_batch(psim_mlp* h, const float* x, const float* t, int rows) {
+    int rc;
+    if ((rc = ensure_batch(h, rows)) != 0) { return rc; }
+    MLP_CHECK(h, hipMemcpyAsync(h->xin, x, sizeof(float) * (size_t)rows * h->n_in, hipMemcpyHostToDevice, h->stream));
+    if (t) { MLP_CHECK(h, hipMemcpyAsync(h->t, t, sizeof(float) * (size_t)rows, hipMemcpyHostToDevice, h->stream)); }
+    return PSIM_OK;
+}
+
+int read_loss(psim_mlp* h, int rows, float* loss) {
+    std::vector<float> terms((size_t)rows);
+    MLP_CHECK(h, hipMemcpyAsync(terms.data(), h->loss, sizeof(float) * (size_t)rows, hipMemcpyDeviceToHost, h->stream));
+    MLP_CHECK(h, hipStreamSynchronize(h->stream));
+    double s = 0.;
+    for (float v : terms) { s += v; }
+    *loss = (float)s;
+    return PSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psim_mlp_create(psim_mlp** out, int device, int32_t n_in, int32_t hidden, int32_t layers) {
+    if (!out || n_in <= 0 || hidden <= 0 || layers <= 0 || n_in > 4096 || hidden > 8192 || layers > 16) {
+        return PSIM_E_INVALID;
+    }
+    *out = nullptr;
+    auto* h = new psim_mlp();
+    h->device = device;
+    h->n_in = n_in;
+    h->hidden = hidden;
+    h->layers = layers;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess) {
+        delete h;
+        return PSIM_E_DEVICE;
+    }
+    h->dims.push_back(n_in);
+    for (int l = 0; l < layers; ++l) { h->dims.push_back(hidden); }
+    h->dims.push_back(PAD_OUT);
+    int64_t off = 0;
+    for (size_t l = 0; l + 1 < h->dims.size(); ++l) {
+        h->w_off.push_back(off);
+        off += (int64_t)h->dims[l] * h->dims[l + 1];
+        h->b_off.push_back(off);
+        off += h->dims[l + 1];
+    }
+    h->n_params = off;
+    h->p = mlp_alloc<float>((size_t)off);
+    h->g = mlp_alloc<float>((size_t)off);
+    h->m1 = mlp_alloc<float>((size_t)off);
+    h->m2 = mlp_alloc<float>((size_t)off);
+    h->mask = mlp_alloc<float>((size_t)off);
+    h->pb = mlp_alloc<uint16_t>((size_t)off);
+    if (!h->p || !h->g || !h->m1 || !h->m2 || !h->mask || !h->pb) {
+        psim_mlp_destroy(h);
+        return PSIM_E_DEVICE;
+    }
+    // mask: 1 for every parameter except the padded output columns (weights and biases)
+    std::vector<float> mask((size_t)off, 1.f);
+    const int L = (int)h->dims.size() - 1;
+    for (int k = 0; k < h->dims[L - 1]; ++k) {
+        for (int n = 1; n < PAD_OUT; ++n) { mask[(size_t)(h->w_off[L - 1] + (int64_t)k * PAD_OUT + n)] = 0.f; }
+    }
+    for (int n = 1; n < PAD_OUT; ++n) { mask[(size_t)(h->b_off[L - 1] + n)] = 0.f; }
+    if (hipMemcpy(h->mask, mask.data(), sizeof(float) * mask.size(), hipMemcpyHostToDevice) != hipSuccess
+        || hipMemset(h->p, 0, sizeof(float) * (size_t)off) != hipSuccess
+        || hipMemset(h->m1, 0, sizeof(float) * (size_t)off) != hipSuccess
+        || hipMemset(h->m2, 0, sizeof(float) * (size_t)off) != hipSuccess
+        || hipMemset(h->pb, 0, sizeof(uint16_t) * (size_t)off) != hipSuccess) {
+        psim_mlp_destroy(h);
+        return PSIM_E_DEVICE;
+    }
+    *out = h;
+    return PSIM_OK;
+}
+
+void psim_mlp_destroy(psim_mlp* h) {
+    if (!h) { return; }
+    (void)hipSetDevice(h->device);
+    if (h->stream) { (void)hipStreamSynchronize(h->stream); }
+    free_batch(h);
+    void* ps[] = { h->p, h->g, h->m1, h->m2, h->mask, h->pb };
+    for (void* q : ps) {
+        if (q) { (void)hipFree(q); }
+    }
+    if (h->stream) { (void)hipStreamDestroy(h->stream); }
+    delete h;
+}
+
+const char* psim_mlp_last_error(const psim_mlp* h) { return h ? h->err.c_str() : "null surrogate"; }
+
+int64_t psim_mlp_num_params(const psim_mlp* h) { return h ? h->n_params : 0; }
+
+int psim_mlp_set_params(psim_mlp* h, const float* params) {
+    if (!h || !params) { return PSIM_E_INVALID; }
+    MLP_CHECK(h, hipSetDevice(h->device));
+    std::vector<float> p(params, params + h->n_params);
+    // padded output columns are forced to zero
+    const int L = (int)h->dims.size() - 1;
+    for (int k = 0; k < h->dims[L - 1]; ++k) {
+        for (int n = 1; n < PAD_OUT; ++n) { p[(size_t)(h->w_off[L - 1] + (int64_t)k * PAD_OUT + n)] = 0.f; }
+    }
+    for (int n = 1; n < PAD_OUT; ++n) { p[(size_t)(h->b_off[L - 1] + n)] = 0.f; }
+    MLP_CHECK(h, hipMemcpy(h->p, p.data(), sizeof(float) * p.size(), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(ew_blocks(h->n_params)), dim3(EW_BLOCK), 0, h->stream, h->p, h->pb,
+        h->n_params);
+    MLP_CHECK(h, hipGetLastError());
+    MLP_CHECK(h, hipMemsetAsync(h->m1, 0, sizeof(float) * (size_t)h->n_params, h->stream));
+    MLP_CHECK(h, hipMemsetAsync(h->m2, 0, sizeof(float) * (size_t)h->n_params, h->stream));
+    MLP_CHECK(h, hipStreamSynchronize(h->stream));
+    h->step = 0;
+    return PSIM_OK;
+}
+
+int psim_mlp_get_params(psim_mlp* h, float* params) {
+    if (!h || !params) { return PSIM_E_INVALID; }
+    MLP_CHECK(h, hipSetDevice(h->device));
+    MLP_CHECK(h, hipStreamSynchronize(h->stream));
+    MLP_CHECK(h, hipMemcpy(params, h->p, sizeof(float) * (size_t)h->n_params, hipMemcpyDeviceToHost));
+    return PSIM_OK;
+}
+
+int psim_mlp_forward(psim_mlp* h, const float* x, int32_t rows, float* y) {
+    if (!h || !x || !y || rows <= 0) { return PSIM_E_INVALID; }
+    MLP_CHECK(h, hipSetDevice(h->device));
+    int rc;
+    if ((rc = load_batch(h, x, nullptr, rows)) != 0) { return rc; }
+    if ((rc = forward(h, rows)) != 0) { return rc; }
+    const int L = (int)h->dims.size() - 1;
+    std::vector<float> z((size_t)rows * PAD_OUT);
+    MLP_CHECK(h, hipMemcpyAsync(z.data(), h->z[L - 1], sizeof(float) * z.size(), hipMemcpyDeviceToHost, h->stream));
+    MLP_CHECK(h, hipStreamSynchronize(h->stream));
+    for (int r = 0; r < rows; ++r) { y[r] = z[(size_t)r * PAD_OUT]; }
+    return PSIM_OK;
+}
+
+int psim_mlp_gradients(psim_mlp* h, const float* x, const float* t, int32_t rows, float* grad, float* loss) {
+    if (!h || !x || !t || rows <= 0) { return PSIM_E_INVALID; }
+    MLP_CHECK(h, hipSetDevice(h->device));
+    int rc;
+    if ((rc = load_batch(h, x, t, rows)) != 0) { return rc; }
+    if ((rc = forward(h, rows)) != 0) { return rc; }
+    if ((rc = backward(h, rows)) != 0) { return rc; }
+    if (loss && (rc = read_loss(h, rows, loss)) != 0) { return rc; }
+    if (grad) {
+        MLP_CHECK(h, hipStreamSynchronize(h->stream));
+        MLP_CHECK(h, hipMemcpy(grad, h->g, sizeof(float) * (size_t)h->n_params, hipMemcpyDeviceToHost));
+    }
+    return PSIM_OK;
+}
+
+int psim_mlp_train_step(psim_mlp* h, const float* x, const float* t, int32_t rows, float lr, float* loss) {
+    if (!h || !x || !t || rows <= 0 || !(lr > 0.f)) { return PSIM_E_INVALID; }
+    MLP_CHECK(h, hipSetDevice(h->device));
+    int rc;
+    if ((rc = load_batch(h, x, t, rows)) != 0) { return rc; }
+    if ((rc = forward(h, rows)) != 0) { return rc; }
+    if ((rc = backward(h, rows)) != 0) { return rc; }
+    ++h->step;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)h->step), c2 = 1.f - std::pow(b2, (float)h->step);
+    hipLaunchKernelGGL(adam_kernel, dim3(ew_blocks(h->n_params)), dim3(EW_BLOCK), 0, h->stream, h->p, h->g, h->m1, h->m2,
+        h->mask, h->pb, h->n_params, lr, b1, b2, 1e-8f, c1, c2);
+    MLP_CHECK(h, hipGetLastError());
+    if (loss) { return read_loss(h, rows, loss); }
+    MLP_CHECK(h, hipStreamSynchronize(h->stream));
+    return PSIM_OK;
+}
+
+// C = op(A) x op(B) for bf16 operands given as FP32 (rounded to bf16 here); exposed so the MFMA
+// lane layout and the transposed loads can be checked against a host product.
+int psim_mlp_gemm_check(int device, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const float* A,
+    const float* B, float* C) {
+    if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0) { return PSIM_E_INVALID; }
+    if (hipSetDevice(device) != hipSuccess) { return PSIM_E_DEVICE; }
+    const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
+    std::vector<uint16_t> ha(na), hb(nb);
+    for (size_t i = 0; i < na; ++i) { ha[i] = f32_to_bf16(A[i]); }
+    for (size_t i = 0; i < nb; ++i) { hb[i] = f32_to_bf16(B[i]); }
+    uint16_t *da = nullptr, *db = nullptr;
+    float* dc = nullptr;
+    int rc = PSIM_OK;
+    if (hipMalloc(&da, na * 2) != hipSuccess || hipMalloc(&db, nb * 2) != hipSuccess
+        || hipMalloc(&dc, nc * 4) != hipSuccess) {
+        rc = PSIM_E_DEVICE;
+    } else if (hipMemcpy(da, ha.data(), na * 2, hipMemcpyHostToDevice) != hipSuccess
+               || hipMemcpy(db, hb.data(), nb * 2, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = PSIM_E_DEVICE;
+    } else {
+        const dim3 grid((unsigned)((N + TILE - 1) / TILE), (unsigned)((M + TILE - 1) / TILE));
+        if (ta && tb) {
+            hipLaunchKernelGGL((gemm_bf16_kernel<true, true>), grid, dim3(GEMM_THREADS), 0, 0, da, db, dc, M, N, K);
+        } else if (ta) {
+            hipLaunchKernelGGL((gemm_bf16_kernel<true, false>), grid, dim3(GEMM_THREADS), 0, 0, da, db, dc, M, N, K);
+        } else if (tb) {
+            hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, dim3(GEMM_THREADS), 0, 0, da, db, dc, M, N, K);
+        } else {
+            hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, dim3(GEMM_THREADS), 0, 0, da, db, dc, M, N, K);
+        }
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess
+            || hipMemcpy(C, dc, nc * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = PSIM_E_DEVICE;
+        }
+    }
+    if (da) { (void)hipFree(da); }
+    if (db) { (void)hipFree(db); }
+    if (dc) { (void)hipFree(dc); }
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,254 @@
+"""Generates the surrogate's training data with the GPU Monte Carlo, trains the network on the GPU,
+and measures the warm start on held-out structures.
+
+  python -m psim_amd.train_surrogate data     --out DIR [--first 0] [--count 20000] [--phonons 100000]
+  python -m psim_amd.train_surrogate train    --out DIR [--epochs 40]      (writes DIR/surrogate.npz)
+  python -m psim_amd.train_surrogate evaluate --out DIR [--weights FILE] [--count 512] [--phonons 8000000]
+
+Training structures are psim_amd.configs.batch_geometry_params(random.Random(TRAIN_SEED + i)),
+held-out ones use EVAL_SEED + i; the two sets share no structure.  Targets are the self-consistent
+steady-state shape u of every sensor (psim_amd.surrogate.steady_state_targets).  The evaluation runs every held-out structure from three starts (cold: every
+sensor at t_eq; linear: linear in arc length between the walls; warm: the network's prediction)
+over 25 ns in 5000 steps of 5 ps, and reports the time to steady state of each: the RMS over
+sensors of the change in 100-step (0.5 ns) window-mean temperature below rel_tol x |t_hot - t_cold|
+for three windows in a row.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import random
+import sys
+import time
+
+import numpy as np
+
+TRAIN_SEED = 1_000_000
+EVAL_SEED = 2_000_000
+MIN_WALL_DIFF = 0.5  # K: structures with nearly equal walls carry no signal
+REL_TOLS = (0.0025, 0.004, 0.005, 0.0075, 0.01)
+HEADLINE_TOL = 0.005
+
+
+def structures(seed0: int, count: int) -> list:
+    from . import configs
+
+    out, i = [], 0
+    while len(out) < count:
+        p = configs.batch_geometry_params(random.Random(seed0 + i))
+        i += 1
+        if abs(p["t_hot"] - p["t_cold"]) >= MIN_WALL_DIFF:
+            out.append(p)
+    return out
+
+
+def log(msg: str) -> None:
+    print(msg, flush=True)
+
+
+def cmd_data(args) -> None:
+    """Targets of training structures [first, first + count), written to targets_<first>.npz."""
+    import psim_amd
+    from . import surrogate
+
+    params = structures(TRAIN_SEED, args.first + args.count)[args.first:]
+    engine = psim_amd.Engine(args.device)
+    t0 = time.perf_counter()
+    u_all, done = [], 0
+    for b0 in range(0, len(params), args.chunk):
+        chunk = params[b0:b0 + args.chunk]
+        _, u = surrogate.steady_state_targets(engine, chunk, args.phonons, seed=args.first + b0)
+        u_all.append(u)
+        done += len(chunk)
+        log(f"data: {args.first + done}/{args.first + len(params)} structures, {time.perf_counter() - t0:.0f} s")
+    u = np.concatenate(u_all)
+    os.makedirs(args.out, exist_ok=True)
+    np.savez_compressed(os.path.join(args.out, f"targets_{args.first:06d}.npz"), u=u, first=args.first,
+                        phonons=args.phonons, n_sensors=np.array([p["n_sensors"] for p in params], np.int32))
+    log(f"data: {u.size} sensor targets in {time.perf_counter() - t0:.0f} s")
+
+
+def load_targets(out: str):
+    """All targets_*.npz of `out`, which must cover training structures 0 .. n - 1 in order:
+    (u, n_sensors)."""
+    import glob
+
+    us, ns, expect = [], [], 0
+    for path in sorted(glob.glob(os.path.join(out, "targets_*.npz"))):
+        with np.load(path, allow_pickle=False) as f:
+            if int(f["first"]) != expect:
+                raise ValueError(f"{path} starts at structure {int(f['first'])}, expected {expect}")
+            us.append(f["u"])
+            ns.append(f["n_sensors"])
+            expect += len(f["n_sensors"])
+    if not us:
+        raise FileNotFoundError(f"no targets_*.npz in {out}")
+    return np.concatenate(us), np.concatenate(ns)
+
+
+def cmd_train(args) -> None:
+    from . import surrogate
+
+    u, n_sensors = load_targets(args.out)
+    params = structures(TRAIN_SEED, len(n_sensors))
+    assert [p["n_sensors"] for p in params] == n_sensors.tolist()
+    x = np.concatenate([surrogate.features(p) for p in params])
+    scale = np.concatenate([np.full(p["n_sensors"], abs(p["t_hot"] - p["t_cold"]) / 2.0) for p in params])
+    # the last 5% of structures are held back to watch for over-fitting
+    n_val = max(1, len(params) // 20)
+    cut = int(n_sensors[:-n_val].sum())
+    net = surrogate.Surrogate(device=args.device, seed=args.seed)
+    t0 = time.perf_counter()
+    hist = []
+    for e0 in range(0, args.epochs, 5):
+        span = min(5, args.epochs - e0)
+        h = net.fit(x[:cut], u[:cut], epochs=span, batch=args.batch, lr=args.lr, seed=args.seed + e0,
+                    schedule=(e0, args.epochs))
+        hist.extend(h)
+        val = net.predict(x[cut:]) - u[cut:]
+        log(f"train: epoch {e0 + span}/{args.epochs} loss {h[-1]:.3e} val rmse(u) {np.sqrt(np.mean(val ** 2)):.4f} "
+            f"val rmse {np.sqrt(np.mean((val * scale[cut:]) ** 2)):.3f} K, {time.perf_counter() - t0:.0f} s")
+    seconds = time.perf_counter() - t0
+    path = os.path.join(args.out, "surrogate.npz")
+    surrogate.save_weights(net, path)
+    # the saved (bf16-rounded) network is the one measured
+    net = surrogate.load_weights(path, args.device)
+    val = net.predict(x[cut:]) - u[cut:]
+    # GEMM flops per row: forward 2ab per layer, weight gradient 2ab, input gradient 2ab (not layer 0)
+    macs = [a * b for a, b in zip(net.dims[:-1], net.dims[1:])]
+    flops = float(cut // args.batch * args.batch) * args.epochs * (6 * sum(macs) - 2 * macs[0])
+    report = {"rows": int(cut), "val_rows": int(x.shape[0] - cut), "epochs": args.epochs, "batch": args.batch,
+              "lr": args.lr, "train_s": seconds, "gemm_flops": flops, "tflops_per_s": flops / seconds / 1e12,
+              "loss_history": hist, "val_rmse_u": float(np.sqrt(np.mean(val ** 2))),
+              "val_rmse_K": float(np.sqrt(np.mean((val * scale[cut:]) ** 2)))}
+    with open(os.path.join(args.out, "train_report.json"), "w") as f:
+        json.dump(report, f, indent=1)
+    log(f"train: done, val rmse {report['val_rmse_K']:.3f} K, {report['tflops_per_s']:.1f} TFLOP/s over training")
+
+
+def relaxation_summary(p: dict, temps: np.ndarray, late: int) -> dict:
+    from . import surrogate
+
+    return {"ttss": {str(c): surrogate.time_to_steady_state(p, temps, c) for c in REL_TOLS},
+            "late_u": surrogate.shape_of(p, temps[:, -late:].mean(axis=1)).tolist()}
+
+
+def evaluate(engine, net, params: list, phonons: int, steps: int, sim_ns: float, seed: int, linear_count: int,
+             progress=log) -> dict:
+    """Time to steady state of `params` from the cold, linear and warm starts; the summary and
+    per-structure rows."""
+    from . import surrogate
+
+    late = steps // 5
+
+    def red(p, temps):
+        return relaxation_summary(p, temps, late)
+
+    t0 = time.perf_counter()
+    warm_inits = surrogate.predict_inits(net, params)
+    predict_s = time.perf_counter() - t0
+    runs = {}
+    for name, inits, ps in (("cold", None, params), ("warm", warm_inits, params),
+                            ("linear", [surrogate.linear_inits(p) for p in params[:linear_count]],
+                             params[:linear_count])):
+        if not ps:
+            continue
+        t1 = time.perf_counter()
+        runs[name] = surrogate.run_relaxations(engine, ps, inits, phonons, steps, sim_ns, seed, red)
+        progress(f"evaluate: {name} start, {len(ps)} structures, {time.perf_counter() - t1:.0f} s")
+    dt_ns = sim_ns / steps
+    summary = {"structures": len(params), "phonons": phonons, "steps": steps, "dt_ps": dt_ns * 1e3,
+               "window_steps": 100, "predict_s": predict_s, "criteria": {}}
+    for c in REL_TOLS:
+        key = str(c)
+        row = {}
+        for name, res in runs.items():
+            t = [r["ttss"][key] for r in res]
+            reached = [v for v in t if v is not None]
+            censored = [steps if v is None else v for v in t]
+            row[name] = {"reached": len(reached), "total_ns": sum(censored) * dt_ns,
+                         "median_ns": float(np.median(censored)) * dt_ns}
+        cold, warm = runs["cold"], runs["warm"]
+        ratios = [(steps if a["ttss"][key] is None else a["ttss"][key]) / (steps if b["ttss"][key] is None
+                                                                             else b["ttss"][key])
+                  for a, b in zip(cold, warm)]
+        row["cold_over_warm_total"] = row["cold"]["total_ns"] / row["warm"]["total_ns"]
+        row["cold_over_warm_median_ratio"] = float(np.median(ratios))
+        if "linear" in runs:
+            n = len(runs["linear"])
+            cold_n = sum(steps if r["ttss"][key] is None else r["ttss"][key] for r in cold[:n])
+            warm_n = sum(steps if r["ttss"][key] is None else r["ttss"][key] for r in warm[:n])
+            lin_n = sum(steps if r["ttss"][key] is None else r["ttss"][key] for r in runs["linear"])
+            row["on_linear_subset"] = {"structures": n, "cold_over_warm": cold_n / warm_n,
+                                       "cold_over_linear": cold_n / lin_n, "linear_over_warm": lin_n / warm_n}
+        summary["criteria"][key] = row
+    # distance of each start from the field its own run settles to (the mean of its last fifth)
+    inits_of = {"cold": [[0.5 * (p["t_hot"] + p["t_cold"])] * p["n_sensors"] for p in params], "warm": warm_inits,
+                "linear": [surrogate.linear_inits(p) for p in params[:linear_count]]}
+    summary["initial_error_rmse_K"] = {}
+    summary["initial_error_rmse_u"] = {}
+    for name, res in runs.items():
+        ek, eu = [], []
+        for p, init, r in zip(params, inits_of[name], res):
+            du = surrogate.shape_of(p, init) - np.asarray(r["late_u"])
+            eu.append(du)
+            ek.append(du * abs(p["t_hot"] - p["t_cold"]) / 2.0)
+        summary["initial_error_rmse_K"][name] = float(np.sqrt(np.mean(np.concatenate(ek) ** 2)))
+        summary["initial_error_rmse_u"][name] = float(np.sqrt(np.mean(np.concatenate(eu) ** 2)))
+    # the settled fields of the cold and warm runs differ by the sensors' table temperatures
+    late_diff = np.concatenate([(np.asarray(w["late_u"]) - np.asarray(c["late_u"])) * abs(p["t_hot"] - p["t_cold"])
+                                / 2.0 for p, c, w in zip(params, runs["cold"], runs["warm"])])
+    summary["late_field_warm_minus_cold_rmse_K"] = float(np.sqrt(np.mean(late_diff ** 2)))
+    rows = [{"length": p["length"], "width": p["width"], "kinked": p["kinked"], "wall_diff": p["t_hot"] - p["t_cold"],
+             "spec": p["spec"], "sensors": p["n_sensors"],
+             **{f"{name}_ttss": runs[name][i]["ttss"] for name in runs if i < len(runs[name])}}
+            for i, p in enumerate(params)]
+    return {"summary": summary, "rows": rows}
+
+
+def cmd_evaluate(args) -> None:
+    import psim_amd
+    from . import surrogate
+
+    params = structures(EVAL_SEED, args.count)
+    engine = psim_amd.Engine(args.device)
+    net = surrogate.load_weights(args.weights or surrogate.WEIGHTS_PATH, args.device)
+    res = evaluate(engine, net, params, args.phonons, 5000, 25.0, seed=7, linear_count=args.linear_count)
+    os.makedirs(args.out, exist_ok=True)
+    with open(os.path.join(args.out, "evaluation.json"), "w") as f:
+        json.dump(res, f, indent=1)
+    log(json.dumps(res["summary"], indent=1))
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("command", choices=("data", "train", "evaluate"))
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--count", type=int, default=None)
+    ap.add_argument("--phonons", type=int, default=None)
+    ap.add_argument("--chunk", type=int, default=1024)
+    ap.add_argument("--first", type=int, default=0)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--weights", default=None)
+    ap.add_argument("--linear-count", type=int, default=128)
+    args = ap.parse_args(argv)
+    if args.command == "data":
+        args.count = args.count or 20000
+        args.phonons = args.phonons or 100_000
+        cmd_data(args)
+    elif args.command == "train":
+        cmd_train(args)
+    else:
+        args.count = args.count or 512
+        args.phonons = args.phonons or 8_000_000
+        cmd_evaluate(args)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

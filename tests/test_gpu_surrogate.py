@@ -1,0 +1,143 @@
+"""The surrogate's BF16-MFMA kernels against CPU references (psim_amd/csrc/surrogate.hip).
+
+- GEMM: every transpose combination on ragged shapes (not multiples of the 64 x 64 tile or of
+  K = 32) against the float64 product of the same bf16-rounded operands.
+- Gradients: the GPU backward against torch autograd in float64 on the CPU, run on the same
+  bf16-rounded weights and inputs, with the forward's roundings reproduced (activations rounded
+  to bf16 between layers; rounding passes gradients straight through).
+- Training: Adam on the GPU fits a smooth function of the inputs.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16(x):
+    return torch.as_tensor(np.asarray(x, np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
+
+
+@pytest.fixture(scope="module")
+def surrogate():
+    from psim_amd import surrogate as s
+    return s
+
+
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_gemm_matches_host_product(surrogate, ta, tb):
+    rng = np.random.default_rng(3 + 2 * ta + tb)
+    M, N, K = 70, 45, 100
+    a = rng.normal(size=(K, M) if ta else (M, K)).astype(np.float32)
+    b = rng.normal(size=(N, K) if tb else (K, N)).astype(np.float32)
+    got = surrogate.gemm_check(a, b, ta, tb)
+    A = _bf16(a).T if ta else _bf16(a)
+    B = _bf16(b).T if tb else _bf16(b)
+    want = A @ B
+    # FP32 accumulation of 100 exact bf16 x bf16 products
+    assert np.max(np.abs(got - want)) < 1e-4 * np.sqrt(K) * np.max(np.abs(want))
+    # a transposed or shifted operand would be far off
+    assert np.corrcoef(got.ravel(), want.ravel())[0, 1] > 0.9999
+
+
+def test_gemm_single_tile_identity(surrogate):
+    """An identity operand returns the other operand (bf16-rounded) exactly: each MFMA output
+    element lands at its own row and column."""
+    rng = np.random.default_rng(9)
+    a = rng.normal(size=(64, 64)).astype(np.float32)
+    eye = np.eye(64, dtype=np.float32)
+    np.testing.assert_array_equal(surrogate.gemm_check(a, eye, False, False), _bf16(a).astype(np.float32))
+    np.testing.assert_array_equal(surrogate.gemm_check(eye, a, False, False), _bf16(a).astype(np.float32))
+
+
+def _reference_gradients(net, params, x, t):
+    """Loss and flat gradient in float64 on the CPU, with the GPU's bf16 roundings in the forward."""
+    views = net.layer_views(params.astype(np.float32))
+    h = torch.tensor(_bf16(x), requires_grad=False)
+    ws, bs = [], []
+    for i, (w, b) in enumerate(views):
+        W = torch.tensor(_bf16(w), requires_grad=True)
+        B = torch.tensor(b.astype(np.float64), requires_grad=True)
+        ws.append(W)
+        bs.append(B)
+        z = h @ W + B
+        if i + 1 < len(views):
+            a = torch.relu(z)
+            # the next layer reads bf16 activations; the rounding is transparent to the gradient
+            h = a + (torch.tensor(_bf16(a.detach().numpy())) - a).detach()
+        else:
+            h = z
+    y = h[:, 0]
+    loss = torch.mean((y - torch.tensor(t, dtype=torch.float64)) ** 2)
+    loss.backward()
+    flat = []
+    for W, B in zip(ws, bs):
+        flat.append(W.grad.numpy().ravel())
+        flat.append(B.grad.numpy().ravel())
+    return float(loss.detach()), np.concatenate(flat)
+
+
+def test_gradients_match_cpu_reference(surrogate):
+    net = surrogate.Surrogate(n_in=32, hidden=96, layers=3, seed=1)
+    rng = np.random.default_rng(5)
+    rows = 150
+    x = rng.normal(size=(rows, 32)).astype(np.float32)
+    t = rng.normal(size=rows).astype(np.float32)
+    params = net.params()
+    loss, g = net.gradients(x, t)
+    ref_loss, ref_g = _reference_gradients(net, params, x, t)
+    assert abs(loss - ref_loss) <= 1e-4 * ref_loss
+    # per layer: the GPU rounds each back-propagated gradient to bf16 (relative 2^-9)
+    off = 0
+    for w, b in net.layer_views(params):
+        for n in (w.size, b.size):
+            got, want = g[off:off + n], ref_g[off:off + n]
+            off += n
+            scale = np.linalg.norm(want)
+            if scale == 0.0:
+                np.testing.assert_array_equal(got, 0.0)
+                continue
+            assert np.linalg.norm(got - want) <= 2e-2 * scale
+    assert off == g.size
+    # the padded output columns get no gradient
+    w_last, b_last = net.layer_views(g)[-1]
+    assert np.all(w_last[:, 1:] == 0.0) and np.all(b_last[1:] == 0.0)
+
+
+def test_gradient_check_rejects_a_wrong_reference(surrogate):
+    """The tolerance above is tight enough to see a gradient computed without the ReLU mask."""
+    net = surrogate.Surrogate(n_in=32, hidden=96, layers=3, seed=1)
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(150, 32)).astype(np.float32)
+    t = rng.normal(size=150).astype(np.float32)
+    _, g = net.gradients(x, t)
+    views = net.layer_views(net.params())
+    # rebuild the first layer's weight gradient with the masks dropped (a linear network)
+    h = [_bf16(x)]
+    for w, b in views[:-1]:
+        h.append(_bf16(np.maximum(h[-1] @ _bf16(w) + b, 0.0)))
+    y = h[-1] @ _bf16(views[-1][0]) + views[-1][1]
+    d = 2.0 * (y[:, 0] - t) / len(t)
+    dz = np.outer(d, np.eye(16)[0])
+    for w, _ in reversed(views[1:]):
+        dz = dz @ _bf16(w).T
+    wrong = h[0].T @ dz
+    got = g[:views[0][0].size].reshape(views[0][0].shape)
+    assert np.linalg.norm(got - wrong) > 2e-2 * np.linalg.norm(wrong)
+
+
+def test_training_fits_a_smooth_function(surrogate):
+    rng = np.random.default_rng(11)
+    x = rng.uniform(-1.0, 1.0, size=(8192, 32)).astype(np.float32)
+    t = (np.sin(2.0 * x[:, 0]) * x[:, 1] + 0.5 * x[:, 2] ** 2).astype(np.float32)
+    net = surrogate.Surrogate(seed=2)
+    base = float(np.mean((t - t.mean()) ** 2))
+    hist = net.fit(x, t, epochs=160, batch=512, lr=1e-3, seed=0)
+    assert hist[-1] < 0.02 * base
+    y = net.predict(x[:512])
+    assert float(np.mean((y - t[:512]) ** 2)) < 0.03 * base
+    # parameters round-trip; prediction depends only on them
+    p = net.params()
+    other = surrogate.Surrogate(seed=99)
+    other.set_params(p)
+    np.testing.assert_array_equal(other.predict(x[:512]), y)
