@@ -34,7 +34,16 @@ U(250, 350) K, specularity U(0, 1), 1e4-1e6 phonons each, deviational steady sta
 second of GPU run time; the same models run one after the other on the GPU, and the reference on
 a sample of them (scaled to 1e4 phonons each), are reported beside it.
 
-Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload lshape|film|batch]
+With --workload surrogate it measures the warm start: held-out structures
+(psim_amd.train_surrogate.structures) run from a cold start and from the trained surrogate's
+prediction (8e6 phonons, 5000 steps of 5 ps, periodic mode); `value` is the ratio of their total
+times to steady state in simulated time (RMS over sensors of the change in 100-step window-mean T
+below 0.005 x |t_hot - t_cold| for 3 windows); cold_over_warm_gpu_seconds is the same ratio in GPU
+time, from runs cut at their measured steady state.  The roofline is the surrogate's BF16 MFMA GEMM kernel against the
+dense bf16 peak, timed with HIP events; its CPU baseline is the reference's rate on two of the
+held-out structures.
+
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload lshape|film|batch|surrogate]
 """
 from __future__ import annotations
 
@@ -216,10 +225,89 @@ def batch_bench(args) -> dict:
     return line
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
+
+
+def surrogate_bench(args) -> dict:
+    """Warm start of held-out structures from the trained surrogate against the cold start, and the
+    surrogate's MFMA GEMM against the bf16 peak."""
+    import numpy as np
+    import psim_amd
+    from psim_amd import configs, surrogate, train_surrogate as ts
+
+    count = args.structures or 128
+    params = ts.structures(ts.EVAL_SEED, count)
+    engine = psim_amd.Engine(0)
+    net = surrogate.load_weights()
+    phonons, steps, sim_ns = 8_000_000, 5000, 25.0
+    t0 = time.perf_counter()
+    res = ts.evaluate(engine, net, params, phonons, steps, sim_ns, seed=7, linear_count=min(32, count),
+                      progress=lambda m: print(m, file=sys.stderr, flush=True))
+    wall = time.perf_counter() - t0
+    s = res["summary"]
+    row = s["criteria"][str(ts.HEADLINE_TOL)]
+    # the GEMM kernel at the training shapes (batch 4096, hidden 512) and at one held-out forward pass
+    rows = sum(p["n_sensors"] for p in params)
+    shapes = {"forward_hidden_b4096": (4096, 512, 512, False, False),
+              "weight_grad_hidden_b4096": (512, 512, 4096, True, False),
+              "input_grad_hidden_b4096": (4096, 512, 512, False, True),
+              f"forward_hidden_{rows}_rows": (rows, 512, 512, False, False),
+              "forward_hidden_65536_rows": (65536, 512, 512, False, False)}
+    gemm = {}
+    for name, (M, N, K, ta, tb) in shapes.items():
+        ms = surrogate.time_gemm(M, N, K, ta, tb, reps=50)
+        gemm[name] = {"M": M, "N": N, "K": K, "ms": ms, "tflops": 2.0 * M * N * K / (ms / 1e3) / 1e12}
+    # one training step (forward, backward, Adam) at batch 4096, timed over 50 steps
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, size=(4096, surrogate.N_FEATURES)).astype(np.float32)
+    y = rng.uniform(-1, 1, size=4096).astype(np.float32)
+    trainer = surrogate.Surrogate(seed=1)
+    trainer.train_step(x, y, 1e-4)
+    t1 = time.perf_counter()
+    for _ in range(50):
+        trainer.train_step(x, y, 1e-4)
+    step_ms = (time.perf_counter() - t1) * 1e3 / 50
+    dom = gemm["forward_hidden_65536_rows"]
+    line = {
+        "metric": "time-to-steady-state cold/warm", "value": row["cold_over_warm_total"], "unit": "x",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": wall * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 network (fp32 accumulate), f64 Monte Carlo",
+        "data": "synthetic",
+        "config": {"workload": f"{count} held-out structures (psim_amd.train_surrogate.structures, seed "
+                               f"{ts.EVAL_SEED}), each from a cold start and from the surrogate's prediction, "
+                               f"{phonons} phonons, {steps} steps of {sim_ns / steps * 1e3:g} ps, periodic mode",
+                   "criterion": f"RMS over sensors of the change in 100-step window-mean T < {ts.HEADLINE_TOL} x "
+                                f"|t_hot - t_cold| for 3 windows",
+                   "network": "32 -> 512 x 4 (ReLU) -> 1, psim_amd/data/surrogate_32x512x4.npz",
+                   "parallelism": "single GPU"},
+        "cold_over_warm_gpu_seconds": s["gpu_seconds_to_steady_state"]["cold_over_warm"],
+        "time_to_steady_state": s,
+        "roofline": {"bound": "mfma", "kernel": "gemm_bf16_kernel", "shape": "65536 x 512 x 512 (forward, hidden)",
+                     "achieved": dom["tflops"], "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": dom["tflops"] / MFMA_BF16_PEAK_TFLOPS, "traffic": None},
+        "gemm": gemm,
+        "training_step_ms_b4096": step_ms,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        sample = [configs.geometry_from_params(dict(p, phonons=20_000), steps, sim_ns, sim_type=0) for p in params[:2]]
+        rates = [reference_rate(c) for c in sample]
+        counted = sum(r[2] for r in rates)
+        wall_cpu = sum(r[1] for r in rates)
+        line["cpu_baseline"] = {"value": counted / wall_cpu, "unit": "phonon-steps/s", "cores": 1,
+                                "kind": rates[0][3],
+                                "sample": "the first 2 held-out structures, cold start, 2e4 phonons each, "
+                                          f"{steps} steps of {sim_ns / steps * 1e3:g} ps, steady-state mode, one "
+                                          "process; phonon-steps counted by the oracle"}
+    engine.close()
+    return line
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["lshape", "film", "batch"], default="lshape")
+    ap.add_argument("--workload", choices=["lshape", "film", "batch", "surrogate"], default="lshape")
+    ap.add_argument("--structures", type=int, default=None, help="held-out structures (surrogate workload)")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--phonons", type=int, default=None)
@@ -247,6 +335,9 @@ def main() -> int:
 
     if args.workload == "batch":
         print(json.dumps(batch_bench(args)), flush=True)
+        return 0
+    if args.workload == "surrogate":
+        print(json.dumps(surrogate_bench(args)), flush=True)
         return 0
 
     import psim_amd

@@ -47,6 +47,11 @@ int psim_mlp_train_step(psim_mlp* mlp, const float* x, const float* t, int32_t r
 int psim_mlp_gemm_check(int device, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const float* A,
     const float* B, float* C);
 
+/* Average duration (ms) of one launch of that GEMM kernel on op(A)[M][K] x op(B)[K][N], from HIP
+ * events around `reps` launches on a stream of its own: the measurement behind its roofline. */
+int psim_mlp_time_gemm(int device, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, int32_t reps,
+    float* ms_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

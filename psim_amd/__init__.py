@@ -98,7 +98,7 @@ def exported_symbols_of_header(path: str = HEADER_PATH) -> list[str]:
     """Function names declared in include/psim_gpu.h."""
     import re
     text = open(path).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(psim_[a-z_]+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|void|const char\*)\s+(psim_[a-z_]+)\s*\(", text, re.M)))
 
 
 def _p(a, t):

@@ -255,6 +255,18 @@ struct Lists {
     int32_t window;
 };
 
+// Tally path of a transport launch, fixed at compile time so that each variant carries only its
+// own code: none (a window before the first recorded step), LDS-privatised window partials, or
+// FP64 atomics into the global tallies (too many sensors for LDS).
+constexpr int TALLY_NONE = 0, TALLY_LDS = 1, TALLY_GLOBAL = 2;
+
+// Waves per SIMD the compiler must leave room for.  Without tally code the kernel fits 128 VGPRs
+// with a few values spilled to scratch, and four waves per SIMD hide more of its load latency;
+// with the tally code, 128 VGPRs spill in the tally path itself and three waves (~143 VGPRs) are
+// faster.
+#ifndef PSIM_UNTALLIED_WAVES_PER_EU
+#define PSIM_UNTALLIED_WAVES_PER_EU 4
+#endif
 #ifndef PSIM_TRANSPORT_WAVES_PER_EU
 #define PSIM_TRANSPORT_WAVES_PER_EU 2
 #endif
@@ -271,16 +283,16 @@ enum : int { S_FETCH = 0, S_TTS = 1, S_NI = 2, S_COLL = 3, S_SCAT = 4, S_EXIT = 
 // scattering-time draw) run once for many lanes instead of once per lane that needs them.
 // Each lane still performs exactly the reference's sequence of operations for its phonon, so
 // trajectories are unchanged.
-// GLOBAL_TALLY: compile-time choice of the tally path (Tally::global), so that the common LDS
-// path carries no code of the other.
-template <bool GLOBAL_TALLY>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PSIM_TRANSPORT_WAVES_PER_EU)))
+// TALLY: the tally path of this launch (TALLY_NONE, TALLY_LDS, TALLY_GLOBAL).
+template <int TALLY>
+__global__ void __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(TALLY == TALLY_NONE ? PSIM_UNTALLIED_WAVES_PER_EU : PSIM_TRANSPORT_WAVES_PER_EU)))
 transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const uint64_t* __restrict__ seed_p,
     Tally tally, unsigned long long* __restrict__ counters, Fate fate) {
     const uint64_t seed = *seed_p;
     extern __shared__ unsigned char lds_raw[];
     const int S = m.n_sensors;
-    const int WS = (tally.rec && !GLOBAL_TALLY) ? tally.W * S : 0;
+    const int WS = (TALLY == TALLY_LDS && tally.rec) ? tally.W * S : 0;
     double* lfx = reinterpret_cast<double*>(lds_raw);
     double* lfy = lfx + WS;
     int32_t* lcnt = reinterpret_cast<int32_t*>(lfy + WS);
@@ -356,11 +368,11 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
             events = 0;
             if (++p.step < m.N) {
                 psim_stream_set_step(&st, (uint32_t)p.step);
-                if (p.step >= m.adj) {  // Sensor::updateHeatParams, sensor.cpp:43-52
+                if (TALLY != TALLY_NONE && p.step >= m.adj) {  // Sensor::updateHeatParams, sensor.cpp:43-52
                     const int idx = p.step - m.adj;
                     if (idx < m.R && tally.rec) {
                         const double sg = (double)p.sign;
-                        if constexpr (GLOBAL_TALLY) {
+                        if constexpr (TALLY == TALLY_GLOBAL) {
                             const int64_t t = (int64_t)cur_sensor * m.R + idx;
                             atomicAdd(reinterpret_cast<unsigned long long*>(&tally.counts[t]),
                                 (unsigned long long)(int64_t)p.sign);
@@ -707,26 +719,32 @@ __global__ void __launch_bounds__(BLOCK) zero_words_kernel(uint32_t* __restrict_
 
 // Sums the block partials of one window into the tallies: entry e = w * S + s of window step
 // k0 + 1 + w goes to tally index (k0 + 1 + w) - adj of sensor s.
+// Sums the per-block window partials into the tallies.  Block (x, y) adds transport blocks
+// [y * per, (y + 1) * per) of window entries [x * BLOCK, (x + 1) * BLOCK) (e = step-in-window * S +
+// sensor) and adds its sums with atomics: enough blocks to fill the chip even for a window of a
+// few sensors.  Counts stay exact (integer atomics); flux sums in another order, as the LDS
+// atomics of the transport kernel already do.
 __global__ void __launch_bounds__(BLOCK) reduce_kernel(const int32_t* __restrict__ pcnt,
-    const double* __restrict__ pfx, const double* __restrict__ pfy, int n_blocks, int WS, int S, int64_t k0,
+    const double* __restrict__ pfx, const double* __restrict__ pfy, int n_blocks, int per, int WS, int S, int64_t k0,
     int64_t adj, int64_t R, int64_t* __restrict__ counts, double* __restrict__ flux) {
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     if (e >= WS) { return; }
     const int w = e / S, s = e % S;
     const int64_t idx = k0 + 1 + w - adj;
     if (idx < 0 || idx >= R) { return; }
+    const int g0 = blockIdx.y * per, g1 = min(n_blocks, g0 + per);
     int64_t cnt = 0;
     double fx = 0., fy = 0.;
-    for (int g = 0; g < n_blocks; ++g) {
+    for (int g = g0; g < g1; ++g) {
         const int64_t k = (int64_t)g * WS + e;
         cnt += pcnt[k];
         fx += pfx[k];
         fy += pfy[k];
     }
     const int64_t t = (int64_t)s * R + idx;
-    counts[t] += cnt;
-    flux[2 * t] += fx;
-    flux[2 * t + 1] += fy;
+    if (cnt != 0) { atomicAdd(reinterpret_cast<unsigned long long*>(counts + t), (unsigned long long)cnt); }
+    if (fx != 0.) { atomicAdd(flux + 2 * t, fx); }
+    if (fy != 0.) { atomicAdd(flux + 2 * t + 1, fy); }
 }
 
 template <typename T>
@@ -1104,19 +1122,26 @@ int enqueue_run(psim_gpu_ctx* c, const RunPlan& plan, bool eager) {
         Tally t{ c->d_pcnt, c->d_pfx, c->d_pfy, c->d_counts, c->d_flux, rec ? 1 : 0, global_tally ? 1 : 0, W };
         const size_t lds = (rec && !global_tally) ? (size_t)WS * 20 : 0;
         if (eager) { HIP_CHECK(c, hipEventRecord(c->ev[1], c->stream)); }
-        if (global_tally) {
-            hipLaunchKernelGGL(transport_kernel<true>, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool, lists,
-                (int32_t)k0, (int32_t)k1, c->d_seed, t, c->d_counters, fate);
+        if (!rec) {  // no step of this window is recorded
+            hipLaunchKernelGGL(transport_kernel<TALLY_NONE>, dim3(grid), dim3(BLOCK), 0, c->stream, c->m, c->pool,
+                lists, (int32_t)k0, (int32_t)k1, c->d_seed, t, c->d_counters, fate);
+        } else if (global_tally) {
+            hipLaunchKernelGGL(transport_kernel<TALLY_GLOBAL>, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool,
+                lists, (int32_t)k0, (int32_t)k1, c->d_seed, t, c->d_counters, fate);
         } else {
-            hipLaunchKernelGGL(transport_kernel<false>, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool, lists,
-                (int32_t)k0, (int32_t)k1, c->d_seed, t, c->d_counters, fate);
+            hipLaunchKernelGGL(transport_kernel<TALLY_LDS>, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool,
+                lists, (int32_t)k0, (int32_t)k1, c->d_seed, t, c->d_counters, fate);
         }
         HIP_CHECK(c, hipGetLastError());
         if (eager) { HIP_CHECK(c, hipEventRecord(c->ev[2], c->stream)); }
         if (rec && !global_tally) {
             const int rblocks = (int)((WS + BLOCK - 1) / BLOCK);
-            hipLaunchKernelGGL(reduce_kernel, dim3(rblocks), dim3(BLOCK), 0, c->stream, c->d_pcnt, c->d_pfx, c->d_pfy,
-                grid, (int)WS, S, k0, P.step_adjustment, R, c->d_counts, c->d_flux);
+            // split the block partials so that the reduce has ~4 blocks per CU
+            const int splits = std::max(1, std::min(grid, (4 * c->n_cu + rblocks - 1) / rblocks));
+            const int per = (grid + splits - 1) / splits;
+            hipLaunchKernelGGL(reduce_kernel, dim3(rblocks, (unsigned)((grid + per - 1) / per)), dim3(BLOCK), 0,
+                c->stream, c->d_pcnt, c->d_pfx, c->d_pfy, grid, per, (int)WS, S, k0, P.step_adjustment, R, c->d_counts,
+                c->d_flux);
             HIP_CHECK(c, hipGetLastError());
         }
         if (w + 1 < (int32_t)n_windows) {  // survivors back into live[0], in cell order

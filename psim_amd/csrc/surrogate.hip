@@ -22,9 +22,8 @@
 
 namespace {
 
-constexpr int TILE = 64;  // output tile of a GEMM block
-constexpr int KSTEP = 32;  // K of one MFMA
-constexpr int GEMM_THREADS = 256;  // 4 waves; wave w computes rows [16 w, 16 w + 16) of the tile
+constexpr int GEMM_THREADS = 256;  // 4 waves
+constexpr int BATCH_ROUND = 64;  // batch buffers are sized in multiples of this many rows
 constexpr int PAD_OUT = 16;  // output-layer columns (1 used)
 constexpr int EW_BLOCK = 256;
 
@@ -49,47 +48,154 @@ __host__ __device__ inline float bf16_to_f32(uint16_t h) {
 
 // C[M][N] = op(A)[M][K] x op(B)[K][N] in FP32.  Stored operands (bf16 bits):
 //   A: TA ? [K][M] : [M][K];  B: TB ? [N][K] : [K][N].
-// Both operands are staged through LDS K-contiguous (As[m][k], Bs[n][k]), so a lane's eight K
-// values of its MFMA fragment are one 16-byte read.  Out-of-range elements load as zero.
-template <bool TA, bool TB>
+// A block of 256 threads (4 waves) computes a BM x BN tile; each wave a WM x WN sub-tile of
+// (WM / 16) x (WN / 16) MFMA 16x16x32 accumulators.  Per K step of 32, every thread moves 16-byte
+// vectors of both operands from HBM into registers while the waves run the MFMAs of the previous
+// step, then into the other of two LDS buffers, K-contiguous (As[m][k], Bs[n][k]), so a lane's
+// eight K values of its MFMA fragment are one 16-byte LDS read.  One barrier per step: a buffer is
+// refilled only after every wave has passed the barrier that ends the step reading it.
+// Vectors that cross the matrix edge (or operands whose rows are not 16-byte aligned: vec_a /
+// vec_b = 0) load element by element; out-of-range elements load as zero.
+constexpr int BK = 32;
+constexpr int LDK = BK + 8;  // LDS row pitch in bf16 (80 bytes: 16-byte aligned, staggers banks)
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <bool TA, bool TB, int BM, int BN, int WM, int WN>
 __global__ void __launch_bounds__(GEMM_THREADS) gemm_bf16_kernel(const uint16_t* __restrict__ A,
-    const uint16_t* __restrict__ B, float* __restrict__ C, int M, int N, int K) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[TILE][KSTEP + 8];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[TILE][KSTEP + 8];
+    const uint16_t* __restrict__ B, float* __restrict__ C, int M, int N, int K, int vec_a, int vec_b) {
+    constexpr int WAVES_N = BN / WN;
+    constexpr int TM = WM / 16, TN = WN / 16;
+    constexpr int AV = BM * BK / 8 / GEMM_THREADS;  // 16-byte vectors per thread per step
+    constexpr int BV = BN * BK / 8 / GEMM_THREADS;
+    static_assert((BM / WM) * (BN / WN) == GEMM_THREADS / 64, "one sub-tile per wave");
+    static_assert(AV >= 1 && BV >= 1, "tile too small for the thread count");
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][BM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN][LDK];
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
-    const int m0 = blockIdx.y * TILE, n0 = blockIdx.x * TILE;
-    f32x4 acc[4];
-    for (int c = 0; c < 4; ++c) { acc[c] = f32x4{ 0.f, 0.f, 0.f, 0.f }; }
-    for (int k0 = 0; k0 < K; k0 += KSTEP) {
-        // each thread stages 8 elements of each operand tile (64 x 32)
-        for (int e = tid; e < TILE * KSTEP; e += GEMM_THREADS) {
-            const int r = e / KSTEP, kk = e % KSTEP;  // tile row (m or n), k within the step
-            const int m = m0 + r, n = n0 + r, k = k0 + kk;
-            uint16_t a = 0, b = 0;
-            if (m < M && k < K) { a = TA ? A[(int64_t)k * M + m] : A[(int64_t)m * K + k]; }
-            if (n < N && k < K) { b = TB ? B[(int64_t)n * K + k] : B[(int64_t)k * N + n]; }
-            As[r][kk] = a;
-            Bs[r][kk] = b;
+    const int wm0 = (wave / WAVES_N) * WM, wn0 = (wave % WAVES_N) * WN;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    u32x4 ra[AV], rb[BV];
+
+    // One operand tile (rows r0 .. r0 + R of the output dimension, K step k0) into registers.
+    //   kmajor: stored [row][K] (K contiguous) -> vector v covers row v / 4, k (v % 4) * 8 .. + 8
+    //   else:   stored [K][row] (rows contiguous) -> vector v covers k v / (R / 8), rows (v % (R / 8)) * 8 .. + 8
+    auto load = [&](const uint16_t* __restrict__ X, int rows, int r0, int k0, bool kmajor, int vec, u32x4* reg,
+                    int count, int R) {
+        for (int i = 0; i < count; ++i) {
+            const int v = tid + i * GEMM_THREADS;
+            int r, k;
+            if (kmajor) {
+                r = r0 + v / (BK / 8);
+                k = k0 + (v % (BK / 8)) * 8;
+            } else {
+                k = k0 + v / (R / 8);
+                r = r0 + (v % (R / 8)) * 8;
+            }
+            const bool full = kmajor ? (r < rows && k + 8 <= K) : (k < K && r + 8 <= rows);
+            if (vec && full) {
+                const int64_t off = kmajor ? (int64_t)r * K + k : (int64_t)k * rows + r;
+                reg[i] = *reinterpret_cast<const u32x4*>(X + off);
+            } else {
+                uint16_t e[8];
+                for (int j = 0; j < 8; ++j) {
+                    const int rr = kmajor ? r : r + j, kk = kmajor ? k + j : k;
+                    e[j] = (rr < rows && kk < K) ? X[kmajor ? (int64_t)rr * K + kk : (int64_t)kk * rows + rr] : 0;
+                }
+                std::memcpy(&reg[i], e, 16);
+            }
         }
-        __syncthreads();
-        const int kq = (lane / 16) * 8;
-        bf16x8 fa;
-        std::memcpy(&fa, &As[wave * 16 + lane % 16][kq], 16);
-        for (int c = 0; c < 4; ++c) {
-            bf16x8 fb;
-            std::memcpy(&fb, &Bs[c * 16 + lane % 16][kq], 16);
-            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc[c], 0, 0, 0);
+    };
+    auto store = [&](uint16_t (*S)[LDK], bool kmajor, const u32x4* reg, int count, int R) {
+        for (int i = 0; i < count; ++i) {
+            const int v = tid + i * GEMM_THREADS;
+            if (kmajor) {
+                std::memcpy(&S[v / (BK / 8)][(v % (BK / 8)) * 8], &reg[i], 16);
+            } else {
+                uint16_t e[8];
+                std::memcpy(e, &reg[i], 16);
+                const int k = v / (R / 8), r = (v % (R / 8)) * 8;
+                for (int j = 0; j < 8; ++j) { S[r + j][k] = e[j]; }
+            }
+        }
+    };
+
+    f32x4 acc[TM][TN];
+    for (int i = 0; i < TM; ++i) {
+        for (int j = 0; j < TN; ++j) { acc[i][j] = f32x4{ 0.f, 0.f, 0.f, 0.f }; }
+    }
+    const int nk = (K + BK - 1) / BK;
+    load(A, M, m0, 0, !TA, vec_a, ra, AV, BM);
+    load(B, N, n0, 0, TB, vec_b, rb, BV, BN);
+    store(As[0], !TA, ra, AV, BM);
+    store(Bs[0], TB, rb, BV, BN);
+    __syncthreads();
+    const int kq = (lane / 16) * 8;
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) {
+            load(A, M, m0, (kt + 1) * BK, !TA, vec_a, ra, AV, BM);
+            load(B, N, n0, (kt + 1) * BK, TB, vec_b, rb, BV, BN);
+        }
+        bf16x8 fa[TM], fb[TN];
+        for (int i = 0; i < TM; ++i) { std::memcpy(&fa[i], &As[buf][wm0 + i * 16 + lane % 16][kq], 16); }
+        for (int j = 0; j < TN; ++j) { std::memcpy(&fb[j], &Bs[buf][wn0 + j * 16 + lane % 16][kq], 16); }
+        for (int i = 0; i < TM; ++i) {
+            for (int j = 0; j < TN; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            }
+        }
+        if (kt + 1 < nk) {
+            store(As[buf ^ 1], !TA, ra, AV, BM);
+            store(Bs[buf ^ 1], TB, rb, BV, BN);
         }
         __syncthreads();
     }
-    // D fragment: lane holds column lane % 16, rows 4 (lane / 16) + i
-    for (int c = 0; c < 4; ++c) {
-        const int n = n0 + c * 16 + lane % 16;
-        for (int i = 0; i < 4; ++i) {
-            const int m = m0 + wave * 16 + (lane / 16) * 4 + i;
-            if (m < M && n < N) { C[(int64_t)m * N + n] = acc[c][i]; }
+    // D fragment: lane holds column lane % 16, rows 4 (lane / 16) + e
+    for (int i = 0; i < TM; ++i) {
+        for (int j = 0; j < TN; ++j) {
+            const int n = n0 + wn0 + j * 16 + lane % 16;
+            for (int e = 0; e < 4; ++e) {
+                const int m = m0 + wm0 + i * 16 + (lane / 16) * 4 + e;
+                if (m < M && n < N) { C[(int64_t)m * N + n] = acc[i][j][e]; }
+            }
         }
+    }
+}
+
+// Tile shape of a product: 128 x 128 blocks (64 x 64 per wave) when they fill the chip several
+// times over, else 64 x 64 blocks (32 x 32 per wave), which give the batch-sized products of
+// training enough blocks.
+constexpr int BIG_TILE_MIN_BLOCKS = 1024;
+
+template <bool TA, bool TB>
+void launch_gemm_t(const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K, hipStream_t st) {
+    const auto aligned = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+    const int vec_a = aligned(A) && ((TA ? M : K) % 8 == 0);
+    const int vec_b = aligned(B) && ((TB ? K : N) % 8 == 0);
+    const int64_t big = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
+    if (big >= BIG_TILE_MIN_BLOCKS) {
+        const dim3 grid((unsigned)((N + 127) / 128), (unsigned)((M + 127) / 128));
+        hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, 128, 128, 64, 64>), grid, dim3(GEMM_THREADS), 0, st, A, B, C,
+            M, N, K, vec_a, vec_b);
+    } else {
+        const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
+        hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, 64, 64, 32, 32>), grid, dim3(GEMM_THREADS), 0, st, A, B, C, M,
+            N, K, vec_a, vec_b);
+    }
+}
+
+void launch_gemm(bool ta, bool tb, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
+    hipStream_t st) {
+    if (ta && tb) {
+        launch_gemm_t<true, true>(A, B, C, M, N, K, st);
+    } else if (ta) {
+        launch_gemm_t<true, false>(A, B, C, M, N, K, st);
+    } else if (tb) {
+        launch_gemm_t<false, true>(A, B, C, M, N, K, st);
+    } else {
+        launch_gemm_t<false, false>(A, B, C, M, N, K, st);
     }
 }
 
@@ -223,7 +329,7 @@ void free_batch(psim_mlp* h) {
 int ensure_batch(psim_mlp* h, int rows) {
     if (rows <= h->cap) { return PSIM_OK; }
     free_batch(h);
-    const int cap = ((rows + TILE - 1) / TILE) * TILE;
+    const int cap = ((rows + BATCH_ROUND - 1) / BATCH_ROUND) * BATCH_ROUND;
     const int L = (int)h->dims.size() - 1;
     for (int l = 0; l < L; ++l) { h->z.push_back(mlp_alloc<float>((size_t)cap * h->dims[l + 1])); }
     for (int l = 0; l <= L; ++l) { h->a.push_back(mlp_alloc<uint16_t>((size_t)cap * h->dims[l])); }
@@ -248,8 +354,7 @@ int ensure_batch(psim_mlp* h, int rows) {
 
 template <bool TA, bool TB>
 int gemm(psim_mlp* h, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K) {
-    const dim3 grid((unsigned)((N + TILE - 1) / TILE), (unsigned)((M + TILE - 1) / TILE));
-    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB>), grid, dim3(GEMM_THREADS), 0, h->stream, A, B, C, M, N, K);
+    launch_gemm(TA, TB, A, B, C, M, N, K, h->stream);
     MLP_CHECK(h, hipGetLastError());
     return PSIM_OK;
 }
@@ -487,16 +592,7 @@ int psim_mlp_gemm_check(int device, int32_t ta, int32_t tb, int32_t M, int32_t N
                || hipMemcpy(db, hb.data(), nb * 2, hipMemcpyHostToDevice) != hipSuccess) {
         rc = PSIM_E_DEVICE;
     } else {
-        const dim3 grid((unsigned)((N + TILE - 1) / TILE), (unsigned)((M + TILE - 1) / TILE));
-        if (ta && tb) {
-            hipLaunchKernelGGL((gemm_bf16_kernel<true, true>), grid, dim3(GEMM_THREADS), 0, 0, da, db, dc, M, N, K);
-        } else if (ta) {
-            hipLaunchKernelGGL((gemm_bf16_kernel<true, false>), grid, dim3(GEMM_THREADS), 0, 0, da, db, dc, M, N, K);
-        } else if (tb) {
-            hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, dim3(GEMM_THREADS), 0, 0, da, db, dc, M, N, K);
-        } else {
-            hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, dim3(GEMM_THREADS), 0, 0, da, db, dc, M, N, K);
-        }
+        launch_gemm(ta != 0, tb != 0, da, db, dc, M, N, K, 0);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess
             || hipMemcpy(C, dc, nc * 4, hipMemcpyDeviceToHost) != hipSuccess) {
             rc = PSIM_E_DEVICE;
@@ -505,6 +601,56 @@ int psim_mlp_gemm_check(int device, int32_t ta, int32_t tb, int32_t M, int32_t N
     if (da) { (void)hipFree(da); }
     if (db) { (void)hipFree(db); }
     if (dc) { (void)hipFree(dc); }
+    return rc;
+}
+
+// Average duration of one launch of the network's GEMM kernel for op(A)[M][K] x op(B)[K][N], from
+// HIP events around `reps` launches on a stream of its own (operands filled on the device).
+int psim_mlp_time_gemm(int device, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, int32_t reps,
+    float* ms_per_launch) {
+    if (!ms_per_launch || M <= 0 || N <= 0 || K <= 0 || reps <= 0) { return PSIM_E_INVALID; }
+    if (hipSetDevice(device) != hipSuccess) { return PSIM_E_DEVICE; }
+    const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
+    uint16_t *da = nullptr, *db = nullptr;
+    float *dc = nullptr, *src = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = PSIM_OK;
+    const size_t nsrc = std::max(na, nb);
+    if (hipMalloc(&da, na * 2) != hipSuccess || hipMalloc(&db, nb * 2) != hipSuccess
+        || hipMalloc(&dc, nc * 4) != hipSuccess || hipMalloc(&src, nsrc * 4) != hipSuccess
+        || hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess
+        || hipEventCreate(&e1) != hipSuccess) {
+        rc = PSIM_E_DEVICE;
+    } else {
+        std::vector<float> h(nsrc);
+        for (size_t i = 0; i < nsrc; ++i) { h[i] = (float)((i * 2654435761u) % 1000) * 1e-3f - 0.5f; }
+        auto launch = [&] { launch_gemm(ta != 0, tb != 0, da, db, dc, M, N, K, st); };
+        float ms = 0.f;
+        if (hipMemcpy(src, h.data(), nsrc * 4, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = PSIM_E_DEVICE;
+        } else {
+            hipLaunchKernelGGL(to_bf16_kernel, dim3(ew_blocks((int64_t)na)), dim3(EW_BLOCK), 0, st, src, da, (int64_t)na);
+            hipLaunchKernelGGL(to_bf16_kernel, dim3(ew_blocks((int64_t)nb)), dim3(EW_BLOCK), 0, st, src, db, (int64_t)nb);
+            launch();
+            launch();
+            (void)hipEventRecord(e0, st);
+            for (int r = 0; r < reps; ++r) { launch(); }
+            (void)hipEventRecord(e1, st);
+            if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess
+                || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = PSIM_E_DEVICE;
+            }
+        }
+        *ms_per_launch = ms / (float)reps;
+    }
+    if (e0) { (void)hipEventDestroy(e0); }
+    if (e1) { (void)hipEventDestroy(e1); }
+    if (st) { (void)hipStreamDestroy(st); }
+    void* bufs[] = { da, db, dc, src };
+    for (void* q : bufs) {
+        if (q) { (void)hipFree(q); }
+    }
     return rc;
 }
 

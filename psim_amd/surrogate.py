@@ -25,6 +25,7 @@ from . import PsimError, configs, lib
 N_FEATURES = 32
 HIDDEN = 512
 LAYERS = 4
+MAX_BATCH_PHONONS = 400_000_000  # phonon budget of one batched run (~120 GB of HBM)
 WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_32x512x4.npz")
 
 _VP = ctypes.c_void_p
@@ -48,6 +49,8 @@ def _bind():
         "psim_mlp_train_step": (ctypes.c_int, [_VP, _F, _F, ctypes.c_int32, ctypes.c_float, _F]),
         "psim_mlp_gemm_check": (ctypes.c_int, [ctypes.c_int, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                ctypes.c_int32, ctypes.c_int32, _F, _F, _F]),
+        "psim_mlp_time_gemm": (ctypes.c_int, [ctypes.c_int, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _F]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -74,6 +77,16 @@ def gemm_check(a: np.ndarray, b: np.ndarray, ta: bool, tb: bool, device: int = 0
     if rc != 0:
         raise PsimError(f"psim_mlp_gemm_check failed ({rc})")
     return c
+
+
+def time_gemm(M: int, N: int, K: int, ta: bool = False, tb: bool = False, reps: int = 50, device: int = 0) -> float:
+    """Average ms of one launch of the network's GEMM kernel on op(A)[M][K] x op(B)[K][N]."""
+    L = _bind()
+    ms = ctypes.c_float()
+    rc = L.psim_mlp_time_gemm(int(device), int(ta), int(tb), int(M), int(N), int(K), int(reps), ctypes.byref(ms))
+    if rc != 0:
+        raise PsimError(f"psim_mlp_time_gemm failed ({rc})")
+    return float(ms.value)
 
 
 class Surrogate:
@@ -256,14 +269,17 @@ def predict_inits(net: Surrogate, params: list) -> list:
 
 
 def run_relaxations(engine, params: list, inits, phonons: int, steps: int, sim_ns: float, seed: int, reduce,
-                    batch: int = 256, threads: int = 16) -> list:
+                    batch: int = 256, threads: int = 16, progress=None) -> list:
     """Runs every structure in periodic mode (every step recorded) from initial sensor temperatures
-    inits[i] (None: every sensor at t_eq, the cold start), `phonons` each, in batches through
-    psim_batch_run; returns reduce(p, temps [S, steps]) of each."""
+    inits[i] (None: every sensor at t_eq, the cold start), `phonons` each, in batches of at most
+    `batch` structures and MAX_BATCH_PHONONS phonons through psim_batch_run; returns
+    reduce(p, temps [S, steps]) of each; progress(message), if given, after every batch."""
     from concurrent.futures import ThreadPoolExecutor
 
     from . import Model, run_batch
 
+    # every phonon of a batch holds ~300 bytes of HBM (emission pool, two live buffers, lists)
+    batch = max(1, min(batch, MAX_BATCH_PHONONS // max(1, phonons)))
     out = []
     with ThreadPoolExecutor(max_workers=threads) as ex:
         for b0 in range(0, len(params), batch):
@@ -275,7 +291,61 @@ def run_relaxations(engine, params: list, inits, phonons: int, steps: int, sim_n
             run_batch(engine, models, seed + b0)
             out.extend(reduce(p, m.results(0).temps) for p, m in zip(chunk, models))
             del models
+            if progress:
+                progress(f"  batch done: {len(out)}/{len(params)} structures")
     return out
+
+
+def gpu_seconds_to_steady_state(engine, params: list, inits, ttss: list, phonons: int, steps: int, sim_ns: float,
+                                seed: int, threads: int = 16, progress=None) -> dict:
+    """GPU time to bring every structure to its steady state: each run cut at its own time to
+    steady state ttss[i] (steps; None: the whole horizon), with its phonon budget scaled so the
+    energy per phonon equals that of the full run (the noise of the run whose ttss was measured).
+    Runs of equal length share batches.  Returns {"gpu_s", "phonon_steps", "runs"}."""
+    from collections import defaultdict
+    from concurrent.futures import ThreadPoolExecutor
+
+    from . import Model, run_batch
+
+    dt = sim_ns / steps
+
+    def cut_config(i, n):
+        p = params[i]
+        init = None if inits is None else inits[i]
+        full = configs.geometry_from_params(dict(p, phonons=phonons), steps, sim_ns, sim_type=1, step_interval=100,
+                                            t_init=init)
+        probe = configs.geometry_from_params(dict(p, phonons=phonons), n, n * dt, sim_type=1, step_interval=100,
+                                             t_init=init)
+        eff_full = Model(full).prepare(seed)[0]
+        eff_cut = Model(probe).prepare(seed)[0]
+        n_cut = max(1, int(round(phonons * eff_cut / eff_full)))
+        return configs.geometry_from_params(dict(p, phonons=n_cut), n, n * dt, sim_type=1, step_interval=100,
+                                            t_init=init), n_cut
+
+    groups = defaultdict(list)
+    for i, t in enumerate(ttss):
+        groups[steps if t is None else int(t)].append(i)
+    gpu_s, phonon_steps, runs = 0.0, 0, 0
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        for n, idx in sorted(groups.items()):
+            made = list(ex.map(lambda i: cut_config(i, n), idx))
+            b0 = 0
+            while b0 < len(made):
+                # fill a batch up to the phonon cap
+                b1, total = b0, 0
+                while b1 < len(made) and (b1 == b0 or total + made[b1][1] <= MAX_BATCH_PHONONS):
+                    total += made[b1][1]
+                    b1 += 1
+                models = list(ex.map(Model, [c for c, _ in made[b0:b1]]))
+                counters = run_batch(engine, models, seed + runs)
+                gpu_s += engine.last_timing()["run_ms"] / 1e3
+                phonon_steps += counters["phonon_steps"]
+                runs += 1
+                del models
+                b0 = b1
+            if progress:
+                progress(f"  cut runs of {n} steps: {len(idx)} structures, {gpu_s:.1f} GPU s so far")
+    return {"gpu_s": gpu_s, "phonon_steps": phonon_steps, "runs": runs}
 
 
 def steady_state_targets(engine, params: list, phonons: int = 100_000, seed: int = 0, batch: int = 256):

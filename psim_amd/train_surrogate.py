@@ -4,6 +4,7 @@ and measures the warm start on held-out structures.
   python -m psim_amd.train_surrogate data     --out DIR [--first 0] [--count 20000] [--phonons 100000]
   python -m psim_amd.train_surrogate train    --out DIR [--epochs 40]      (writes DIR/surrogate.npz)
   python -m psim_amd.train_surrogate evaluate --out DIR [--weights FILE] [--count 512] [--phonons 8000000]
+  python -m psim_amd.train_surrogate cost     --out DIR [--weights FILE]   (GPU seconds, from DIR/evaluation.json)
 
 Training structures are psim_amd.configs.batch_geometry_params(random.Random(TRAIN_SEED + i)),
 held-out ones use EVAL_SEED + i; the two sets share no structure.  Targets are the self-consistent
@@ -135,9 +136,10 @@ def relaxation_summary(p: dict, temps: np.ndarray, late: int) -> dict:
 
 
 def evaluate(engine, net, params: list, phonons: int, steps: int, sim_ns: float, seed: int, linear_count: int,
-             progress=log) -> dict:
+             progress=log, cost: bool = True) -> dict:
     """Time to steady state of `params` from the cold, linear and warm starts; the summary and
-    per-structure rows."""
+    per-structure rows.  With `cost`, also the GPU seconds of the cold and warm runs cut at their
+    times to steady state (HEADLINE_TOL)."""
     from . import surrogate
 
     late = steps // 5
@@ -155,7 +157,8 @@ def evaluate(engine, net, params: list, phonons: int, steps: int, sim_ns: float,
         if not ps:
             continue
         t1 = time.perf_counter()
-        runs[name] = surrogate.run_relaxations(engine, ps, inits, phonons, steps, sim_ns, seed, red)
+        runs[name] = surrogate.run_relaxations(engine, ps, inits, phonons, steps, sim_ns, seed, red,
+                                               progress=progress)
         progress(f"evaluate: {name} start, {len(ps)} structures, {time.perf_counter() - t1:.0f} s")
     dt_ns = sim_ns / steps
     summary = {"structures": len(params), "phonons": phonons, "steps": steps, "dt_ps": dt_ns * 1e3,
@@ -200,6 +203,18 @@ def evaluate(engine, net, params: list, phonons: int, steps: int, sim_ns: float,
     late_diff = np.concatenate([(np.asarray(w["late_u"]) - np.asarray(c["late_u"])) * abs(p["t_hot"] - p["t_cold"])
                                 / 2.0 for p, c, w in zip(params, runs["cold"], runs["warm"])])
     summary["late_field_warm_minus_cold_rmse_K"] = float(np.sqrt(np.mean(late_diff ** 2)))
+    if cost:
+        key = str(HEADLINE_TOL)
+        summary["gpu_seconds_to_steady_state"] = {"criterion": HEADLINE_TOL}
+        for name, inits in (("cold", None), ("warm", warm_inits)):
+            t1 = time.perf_counter()
+            c = surrogate.gpu_seconds_to_steady_state(engine, params, inits, [r["ttss"][key] for r in runs[name]],
+                                                      phonons, steps, sim_ns, seed + 1000, progress=progress)
+            summary["gpu_seconds_to_steady_state"][name] = c
+            progress(f"evaluate: {name} runs cut at steady state, {c['gpu_s']:.1f} GPU s, "
+                     f"{time.perf_counter() - t1:.0f} s")
+        g = summary["gpu_seconds_to_steady_state"]
+        g["cold_over_warm"] = g["cold"]["gpu_s"] / g["warm"]["gpu_s"]
     rows = [{"length": p["length"], "width": p["width"], "kinked": p["kinked"], "wall_diff": p["t_hot"] - p["t_cold"],
              "spec": p["spec"], "sensors": p["n_sensors"],
              **{f"{name}_ttss": runs[name][i]["ttss"] for name in runs if i < len(runs[name])}}
@@ -221,9 +236,34 @@ def cmd_evaluate(args) -> None:
     log(json.dumps(res["summary"], indent=1))
 
 
+def cmd_cost(args) -> None:
+    """GPU seconds to steady state of the cold and warm runs of an evaluation already in
+    DIR/evaluation.json (runs cut at their measured times to steady state)."""
+    import psim_amd
+    from . import surrogate
+
+    with open(os.path.join(args.out, "evaluation.json")) as f:
+        res = json.load(f)
+    s = res["summary"]
+    params = structures(EVAL_SEED, len(res["rows"]))
+    engine = psim_amd.Engine(args.device)
+    net = surrogate.load_weights(args.weights or surrogate.WEIGHTS_PATH, args.device)
+    key = str(HEADLINE_TOL)
+    out = {"criterion": HEADLINE_TOL, "structures": len(params)}
+    for name, inits in (("cold", None), ("warm", surrogate.predict_inits(net, params))):
+        ttss = [row[f"{name}_ttss"][key] for row in res["rows"]]
+        out[name] = surrogate.gpu_seconds_to_steady_state(engine, params, inits, ttss, s["phonons"], s["steps"],
+                                                          s["steps"] * s["dt_ps"] / 1e3, seed=1007, progress=log)
+        log(f"cost: {name}: {out[name]['gpu_s']:.1f} GPU s, {out[name]['phonon_steps']:.3e} phonon-steps")
+    out["cold_over_warm"] = out["cold"]["gpu_s"] / out["warm"]["gpu_s"]
+    with open(os.path.join(args.out, "cost.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    log(json.dumps(out))
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("command", choices=("data", "train", "evaluate"))
+    ap.add_argument("command", choices=("data", "train", "evaluate", "cost"))
     ap.add_argument("--out", required=True)
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--count", type=int, default=None)
@@ -243,6 +283,8 @@ def main(argv=None) -> int:
         cmd_data(args)
     elif args.command == "train":
         cmd_train(args)
+    elif args.command == "cost":
+        cmd_cost(args)
     else:
         args.count = args.count or 512
         args.phonons = args.phonons or 8_000_000

@@ -24,10 +24,14 @@ def surrogate():
     return s
 
 
+@pytest.mark.parametrize("shape", [(70, 45, 100), (72, 40, 104), (4100, 4104, 64)],
+                         ids=["ragged", "vector-loads", "large-tiles"])
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
-def test_gemm_matches_host_product(surrogate, ta, tb):
+def test_gemm_matches_host_product(surrogate, ta, tb, shape):
+    """Ragged shapes load element by element, shapes whose rows are multiples of 8 load 16-byte
+    vectors, and products of at least 1024 128 x 128 blocks use the large tile."""
     rng = np.random.default_rng(3 + 2 * ta + tb)
-    M, N, K = 70, 45, 100
+    M, N, K = shape
     a = rng.normal(size=(K, M) if ta else (M, K)).astype(np.float32)
     b = rng.normal(size=(N, K) if tb else (K, N)).astype(np.float32)
     got = surrogate.gemm_check(a, b, ta, tb)

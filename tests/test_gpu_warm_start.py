@@ -1,0 +1,61 @@
+"""What the trained surrogate gains: held-out structures started from its prediction against the
+cold start (every sensor at t_eq), with the Monte Carlo producing every reported field.
+
+Structures come from psim_amd.train_surrogate.structures(EVAL_SEED, n), which the network never
+saw; the weights are psim_amd/data/surrogate_32x512x4.npz (written by psim_amd.train_surrogate).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import psim_amd
+    from psim_amd import surrogate, train_surrogate
+
+    return psim_amd.Engine(0), surrogate.load_weights(), surrogate, train_surrogate
+
+
+def test_warm_start_cuts_time_to_steady_state(setup):
+    engine, net, surrogate, ts = setup
+    params = ts.structures(ts.EVAL_SEED, 24)
+    res = ts.evaluate(engine, net, params, phonons=8_000_000, steps=5000, sim_ns=25.0, seed=11, linear_count=0,
+                      progress=lambda m: print(m, flush=True))
+    s = res["summary"]
+    row = s["criteria"][str(ts.HEADLINE_TOL)]
+    print({k: row[k] for k in ("cold", "warm", "cold_over_warm_total", "cold_over_warm_median_ratio")})
+    print("initial error (K):", s["initial_error_rmse_K"])
+    # every warm run settles, and the total time to steady state drops
+    assert row["warm"]["reached"] == len(params)
+    assert row["cold_over_warm_total"] >= 2.5
+    # the prediction starts each run far closer to the field it settles to than t_eq does
+    assert s["initial_error_rmse_K"]["warm"] < 0.25 * s["initial_error_rmse_K"]["cold"]
+
+
+def test_warm_start_settles_where_the_reference_iteration_goes(setup):
+    """Sensor tables follow the initial temperature (sensorController.cpp:38-50), so the field a run
+    settles to depends on its start; the reference iterates toward tables at the steady temperature
+    (model.cpp:157-176).  A start from the prediction settles where a start from the cold run's
+    settled field (the reference's second iteration) does, within the noise, and both differ from
+    the cold run by the first-order table effect."""
+    engine, net, surrogate, ts = setup
+    params = ts.structures(ts.EVAL_SEED + 50_000, 8)
+    steps, late = 5000, 1000
+
+    def settled(p, temps):
+        return temps[:, -late:].mean(axis=1)
+
+    cold = surrogate.run_relaxations(engine, params, None, 8_000_000, steps, 25.0, 21, settled)
+    second = surrogate.run_relaxations(engine, params, [c.tolist() for c in cold], 8_000_000, steps, 25.0, 22,
+                                       settled)
+    warm = surrogate.run_relaxations(engine, params, surrogate.predict_inits(net, params), 8_000_000, steps, 25.0,
+                                     23, settled)
+    half = [abs(p["t_hot"] - p["t_cold"]) / 2.0 for p in params]
+    d_ws = np.concatenate([(w - s) / h for w, s, h in zip(warm, second, half)])
+    d_sc = np.concatenate([(s - c) / h for s, c, h in zip(second, cold, half)])
+    rms_ws, rms_sc = float(np.sqrt(np.mean(d_ws ** 2))), float(np.sqrt(np.mean(d_sc ** 2)))
+    print(f"settled fields, RMS in units of (t_hot - t_eq): warm - second {rms_ws:.4f}, second - cold {rms_sc:.4f}")
+    assert rms_ws < 0.01
+    assert rms_ws < 0.5 * rms_sc

@@ -1,6 +1,7 @@
 """C ABI and C++17 host (CPU only: nothing here launches a kernel).
 
-- libpsim_amd.so loads and exports every function include/psim_gpu.h declares;
+- libpsim_amd.so loads and exports every function include/psim_gpu.h and
+  include/psim_surrogate.h declare;
 - the host's budget (bounds, tables, energy per phonon, Bernoulli-rounded counts per
   builder) equals the oracle's for several seeds, so the engine and the oracle start from
   the same phonons;
@@ -29,6 +30,10 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(psim_amd.LIB_PATH)
     names = psim_amd.exported_symbols_of_header()
     assert len(names) >= 20
+    surrogate_names = psim_amd.exported_symbols_of_header(os.path.join(os.path.dirname(psim_amd.HEADER_PATH),
+                                                                       "psim_surrogate.h"))
+    assert len(surrogate_names) == 11
+    names += surrogate_names
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing
 
