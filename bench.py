@@ -329,7 +329,7 @@ def main() -> int:
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (the multi-GPU design, sharding "
-              "phonon ids and all-reducing the tallies, is described in DESIGN.md and not run here).",
+              "phonon ids and all-reducing the tallies, is not run here).",
               file=sys.stderr)
         return 2
 

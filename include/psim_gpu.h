@@ -221,6 +221,14 @@ int psim_gpu_set_grid(psim_gpu_ctx* ctx, int32_t blocks_per_cu);
  * std::for_each, modelSimulator.cpp:227-253). */
 int psim_gpu_run_graph(psim_gpu_ctx* ctx, uint64_t seed, int64_t* counts, double* flux, int64_t* counters);
 
+/* The uploaded run replayed from its graph on the caller's HIP stream (a hipStream_t passed as
+ * void*; NULL is the null stream), returning at once.  The results are in the engine's device
+ * buffers once that stream reaches this point: synchronise it, then read them with
+ * psim_gpu_fetch.  One run in flight per context: the runs share the engine's buffers.  For
+ * callers that overlap the engine with their own work on a stream; the reference's loop is
+ * synchronous (ModelSimulator::runSimulation, psim/src/modelSimulator.cpp:39-41). */
+int psim_gpu_run_async(psim_gpu_ctx* ctx, uint64_t seed, void* stream);
+
 /* Autotuner: runs the uploaded model once for each candidate launch shape (untallied window
  * length 0 / 1024 / 256, 8 or 3 transport blocks per CU; none changes any result), keeps the
  * fastest in the context and reports it.  No counterpart in the reference. */

@@ -70,6 +70,7 @@ def lib():
         "psim_gpu_set_grid": (ctypes.c_int, [_VP, ctypes.c_int32]),
         "psim_gpu_run_graph": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64),
                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
+        "psim_gpu_run_async": (ctypes.c_int, [_VP, ctypes.c_uint64, _VP]),
         "psim_gpu_autotune": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32),
                                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
         "psim_model_from_json": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
@@ -153,6 +154,12 @@ class Engine:
         self._check(self._L.psim_gpu_run_graph(self._h, int(seed), _p(counts, _I64), _p(flux, _D),
                                                _p(counters, _I64)))
         return counts, flux, dict(zip(COUNTER_NAMES, counters.tolist()))
+
+    def run_async(self, seed: int, stream: int) -> None:
+        """Enqueues the uploaded run's graph on a HIP stream (its raw handle, e.g.
+        torch.cuda.Stream().cuda_stream) and returns at once; after that stream is synchronised,
+        fetch() reads the results."""
+        self._check(self._L.psim_gpu_run_async(self._h, int(seed), ctypes.c_void_p(int(stream))))
 
     def autotune(self, seed: int = 0) -> dict:
         """Tries the launch shapes of psim_gpu_autotune on the uploaded model and keeps the fastest."""

@@ -719,6 +719,12 @@ __global__ void __launch_bounds__(BLOCK) zero_words_kernel(uint32_t* __restrict_
 
 // Sums the block partials of one window into the tallies: entry e = w * S + s of window step
 // k0 + 1 + w goes to tally index (k0 + 1 + w) - adj of sensor s.
+// Stores the run's seed where the kernels read it; the value travels as a kernel argument, so an
+// asynchronous run needs no host buffer that outlives the call.
+__global__ void seed_kernel(uint64_t* __restrict__ seed_p, uint64_t seed) {
+    if (threadIdx.x == 0) { *seed_p = seed; }
+}
+
 // Sums the per-block window partials into the tallies.  Block (x, y) adds transport blocks
 // [y * per, (y + 1) * per) of window entries [x * BLOCK, (x + 1) * BLOCK) (e = step-in-window * S +
 // sensor) and adds its sums with atomics: enough blocks to fill the chip even for a window of a
@@ -1203,11 +1209,9 @@ int run_impl(psim_gpu_ctx* c, uint64_t seed) {
     return finish_run(c);
 }
 
-// The same run replayed from a graph captured on first use (and again whenever c->gen moves).
-int run_graph_impl(psim_gpu_ctx* c, uint64_t seed) {
-    RunPlan plan;
+// Captures the uploaded run as a graph on first use, and again whenever c->gen moves.
+int ensure_graph(psim_gpu_ctx* c, RunPlan& plan) {
     int rc;
-    if ((rc = prepare_run(c, plan)) != 0) { return rc; }
     if (!c->graph_exec || c->graph_gen != c->gen) {
         if (c->graph_exec) {
             (void)hipGraphExecDestroy(c->graph_exec);
@@ -1231,6 +1235,15 @@ int run_graph_impl(psim_gpu_ctx* c, uint64_t seed) {
         }
         c->graph_gen = c->gen;
     }
+    return PSIM_OK;
+}
+
+// The same run replayed from its captured graph.
+int run_graph_impl(psim_gpu_ctx* c, uint64_t seed) {
+    RunPlan plan;
+    int rc;
+    if ((rc = prepare_run(c, plan)) != 0) { return rc; }
+    if ((rc = ensure_graph(c, plan)) != 0) { return rc; }
     if ((rc = set_seed(c, seed)) != 0) { return rc; }
     HIP_CHECK(c, hipEventRecord(c->ev[0], c->stream));
     HIP_CHECK(c, hipGraphLaunch(c->graph_exec, c->stream));
@@ -1516,6 +1529,23 @@ int psim_gpu_set_grid(psim_gpu_ctx* c, int32_t blocks_per_cu) {
     if (!c || blocks_per_cu < 1 || blocks_per_cu > 64) { return PSIM_E_INVALID; }
     if (c->blocks_per_cu != blocks_per_cu) { c->gen++; }
     c->blocks_per_cu = blocks_per_cu;
+    return PSIM_OK;
+}
+
+int psim_gpu_run_async(psim_gpu_ctx* c, uint64_t seed, void* stream) {
+    if (!c) { return PSIM_E_INVALID; }
+    HIP_CHECK(c, hipSetDevice(c->device));
+    RunPlan plan;
+    int rc;
+    if ((rc = prepare_run(c, plan)) != 0) { return rc; }
+    if ((rc = ensure_graph(c, plan)) != 0) { return rc; }
+    // work the engine queued on its own stream (uploads, clears, a capture) comes first
+    HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(seed_kernel, dim3(1), dim3(64), 0, st, c->d_seed, seed);
+    HIP_CHECK(c, hipGetLastError());
+    HIP_CHECK(c, hipGraphLaunch(c->graph_exec, st));
+    c->run_ms = -1.;  // not timed
     return PSIM_OK;
 }
 

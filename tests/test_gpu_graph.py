@@ -1,4 +1,5 @@
-"""The run captured as a HIP graph (psim_gpu_run_graph) and the autotuner against eager runs.
+"""The run captured as a HIP graph (psim_gpu_run_graph, psim_gpu_run_async) and the autotuner
+against eager runs.
 
 Every replay, with eager runs and synchronisations between the replays, must give the eager
 result of the same seed: integer tallies and counters exactly, flux sums to 1e-9 relative (their
@@ -71,3 +72,40 @@ def test_autotune_keeps_results(engine):
     _same(engine.run_graph(model.num_sensors, model.num_record, 3), reference)
     engine.set_window(0)
     engine.set_grid(8)
+
+
+def _engine_hip_runtime():
+    """The HIP runtime library libpsim_amd.so itself loaded (from /proc/self/maps), so that the
+    stream below belongs to the engine's runtime (torch may carry a runtime of its own)."""
+    import ctypes
+
+    psim_amd.lib()
+    paths = sorted({line.split()[-1] for line in open("/proc/self/maps")
+                    if "libamdhip64" in line and "torch" not in line})
+    assert paths, "the engine's HIP runtime is not loaded"
+    return ctypes.CDLL(paths[0])
+
+
+def test_async_runs_on_a_caller_stream_match_eager(engine):
+    """psim_gpu_run_async enqueues the graph on the caller's stream and returns; after that
+    stream is synchronised the results equal the eager run of the same seed."""
+    import ctypes
+
+    print("case async on a caller stream and on the null stream", flush=True)
+    hip = _engine_hip_runtime()
+    engine.set_window(0)
+    engine.set_grid(8)
+    model = psim_amd.Model(CASES["bar_periodic_lds_tallies"])
+    model.upload(engine, 0)
+    stream = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(stream)) == 0  # default priority
+    try:
+        for seed in (3, 4, 3):
+            engine.run_async(seed, stream.value)
+            assert hip.hipStreamSynchronize(stream) == 0
+            _same(engine.fetch(model.num_sensors, model.num_record), _eager(engine, model, seed))
+    finally:
+        assert hip.hipStreamDestroy(stream) == 0
+    engine.run_async(8, 0)  # the null stream
+    assert hip.hipDeviceSynchronize() == 0
+    _same(engine.fetch(model.num_sensors, model.num_record), _eager(engine, model, 8))
