@@ -248,11 +248,12 @@ def surrogate_bench(args) -> dict:
     row = s["criteria"][str(ts.HEADLINE_TOL)]
     # the GEMM kernel at the training shapes (batch 4096, hidden 512) and at one held-out forward pass
     rows = sum(p["n_sensors"] for p in params)
-    shapes = {"forward_hidden_b4096": (4096, 512, 512, False, False),
+    # the forward reads the transposed weight copy (B stored [N][K]); weight gradients are split over K
+    shapes = {"forward_hidden_b4096": (4096, 512, 512, False, True),
               "weight_grad_hidden_b4096": (512, 512, 4096, True, False),
               "input_grad_hidden_b4096": (4096, 512, 512, False, True),
-              f"forward_hidden_{rows}_rows": (rows, 512, 512, False, False),
-              "forward_hidden_65536_rows": (65536, 512, 512, False, False)}
+              f"forward_hidden_{rows}_rows": (rows, 512, 512, False, True),
+              "forward_hidden_65536_rows": (65536, 512, 512, False, True)}
     gemm = {}
     for name, (M, N, K, ta, tb) in shapes.items():
         ms = surrogate.time_gemm(M, N, K, ta, tb, reps=50)

@@ -63,7 +63,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 template <bool TA, bool TB, int BM, int BN, int WM, int WN>
 __global__ void __launch_bounds__(GEMM_THREADS) gemm_bf16_kernel(const uint16_t* __restrict__ A,
-    const uint16_t* __restrict__ B, float* __restrict__ C, int M, int N, int K, int vec_a, int vec_b) {
+    const uint16_t* __restrict__ B, float* __restrict__ C, int M, int N, int K, int vec_a, int vec_b, int k_split) {
     constexpr int WAVES_N = BN / WN;
     constexpr int TM = WM / 16, TN = WN / 16;
     constexpr int AV = BM * BK / 8 / GEMM_THREADS;  // 16-byte vectors per thread per step
@@ -125,9 +125,13 @@ __global__ void __launch_bounds__(GEMM_THREADS) gemm_bf16_kernel(const uint16_t*
     for (int i = 0; i < TM; ++i) {
         for (int j = 0; j < TN; ++j) { acc[i][j] = f32x4{ 0.f, 0.f, 0.f, 0.f }; }
     }
-    const int nk = (K + BK - 1) / BK;
-    load(A, M, m0, 0, !TA, vec_a, ra, AV, BM);
-    load(B, N, n0, 0, TB, vec_b, rb, BV, BN);
+    // K range of this block: split z of a product split over K (k_split a multiple of BK), whose
+    // partial product goes to its own M x N slice of C
+    const int kb = blockIdx.z * k_split;
+    const int nk = (min(K, kb + k_split) - kb + BK - 1) / BK;
+    C += (int64_t)blockIdx.z * M * N;
+    load(A, M, m0, kb, !TA, vec_a, ra, AV, BM);
+    load(B, N, n0, kb, TB, vec_b, rb, BV, BN);
     store(As[0], !TA, ra, AV, BM);
     store(Bs[0], TB, rb, BV, BN);
     __syncthreads();
@@ -135,8 +139,8 @@ __global__ void __launch_bounds__(GEMM_THREADS) gemm_bf16_kernel(const uint16_t*
     for (int kt = 0; kt < nk; ++kt) {
         const int buf = kt & 1;
         if (kt + 1 < nk) {
-            load(A, M, m0, (kt + 1) * BK, !TA, vec_a, ra, AV, BM);
-            load(B, N, n0, (kt + 1) * BK, TB, vec_b, rb, BV, BN);
+            load(A, M, m0, kb + (kt + 1) * BK, !TA, vec_a, ra, AV, BM);
+            load(B, N, n0, kb + (kt + 1) * BK, TB, vec_b, rb, BV, BN);
         }
         bf16x8 fa[TM], fb[TN];
         for (int i = 0; i < TM; ++i) { std::memcpy(&fa[i], &As[buf][wm0 + i * 16 + lane % 16][kq], 16); }
@@ -166,11 +170,38 @@ __global__ void __launch_bounds__(GEMM_THREADS) gemm_bf16_kernel(const uint16_t*
 
 // Tile shape of a product: 128 x 128 blocks (64 x 64 per wave) when they fill the chip several
 // times over, else 64 x 64 blocks (32 x 32 per wave), which give the batch-sized products of
-// training enough blocks.
+// training enough blocks.  A product with few output blocks and a long K (the weight gradients,
+// K = batch rows) is split over K into partial products in a workspace, then summed in a fixed
+// order (sum_splits_kernel), when the caller provides one.
 constexpr int BIG_TILE_MIN_BLOCKS = 1024;
+constexpr int SPLIT_TARGET_BLOCKS = 512;
+constexpr int SPLIT_MIN_K = 512;  // per split
+constexpr int MAX_SPLITS = 16;
+
+// C[i] = sum over z of ws[z][i], z in order.
+__global__ void sum_splits_kernel(const float* __restrict__ ws, float* __restrict__ C, int splits, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i >= n) { return; }
+    float s = 0.f;
+    for (int z = 0; z < splits; ++z) { s += ws[(int64_t)z * n + i]; }
+    C[i] = s;
+}
+
+unsigned ew_blocks(int64_t n) { return (unsigned)((n + EW_BLOCK - 1) / EW_BLOCK); }
+
+// Number of K splits of a product on 64 x 64 tiles (1: none); ws_elems floats of workspace.
+int gemm_splits(int M, int N, int K, size_t ws_elems) {
+    const int64_t blocks = (int64_t)((M + 63) / 64) * ((N + 63) / 64);
+    if (blocks >= SPLIT_TARGET_BLOCKS || K < 2 * SPLIT_MIN_K) { return 1; }
+    int splits = (int)std::min<int64_t>(MAX_SPLITS, (SPLIT_TARGET_BLOCKS + blocks - 1) / blocks);
+    splits = std::min(splits, K / SPLIT_MIN_K);
+    while (splits > 1 && (size_t)splits * (size_t)M * (size_t)N > ws_elems) { --splits; }
+    return std::max(1, splits);
+}
 
 template <bool TA, bool TB>
-void launch_gemm_t(const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K, hipStream_t st) {
+void launch_gemm_t(const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K, hipStream_t st, float* ws,
+    size_t ws_elems) {
     const auto aligned = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
     const int vec_a = aligned(A) && ((TA ? M : K) % 8 == 0);
     const int vec_b = aligned(B) && ((TB ? K : N) % 8 == 0);
@@ -178,24 +209,51 @@ void launch_gemm_t(const uint16_t* A, const uint16_t* B, float* C, int M, int N,
     if (big >= BIG_TILE_MIN_BLOCKS) {
         const dim3 grid((unsigned)((N + 127) / 128), (unsigned)((M + 127) / 128));
         hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, 128, 128, 64, 64>), grid, dim3(GEMM_THREADS), 0, st, A, B, C,
-            M, N, K, vec_a, vec_b);
-    } else {
+            M, N, K, vec_a, vec_b, K);
+        return;
+    }
+    const int splits = ws ? gemm_splits(M, N, K, ws_elems) : 1;
+    if (splits == 1) {
         const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
         hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, 64, 64, 32, 32>), grid, dim3(GEMM_THREADS), 0, st, A, B, C, M,
-            N, K, vec_a, vec_b);
+            N, K, vec_a, vec_b, K);
+        return;
     }
+    const int k_split = ((K + splits - 1) / splits + BK - 1) / BK * BK;
+    const int used = (K + k_split - 1) / k_split;
+    const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64), (unsigned)used);
+    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, 64, 64, 32, 32>), grid, dim3(GEMM_THREADS), 0, st, A, B, ws, M, N, K,
+        vec_a, vec_b, k_split);
+    const int64_t n = (int64_t)M * N;
+    hipLaunchKernelGGL(sum_splits_kernel, dim3(ew_blocks(n)), dim3(EW_BLOCK), 0, st, ws, C, used, n);
 }
 
 void launch_gemm(bool ta, bool tb, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
-    hipStream_t st) {
+    hipStream_t st, float* ws = nullptr, size_t ws_elems = 0) {
     if (ta && tb) {
-        launch_gemm_t<true, true>(A, B, C, M, N, K, st);
+        launch_gemm_t<true, true>(A, B, C, M, N, K, st, ws, ws_elems);
     } else if (ta) {
-        launch_gemm_t<true, false>(A, B, C, M, N, K, st);
+        launch_gemm_t<true, false>(A, B, C, M, N, K, st, ws, ws_elems);
     } else if (tb) {
-        launch_gemm_t<false, true>(A, B, C, M, N, K, st);
+        launch_gemm_t<false, true>(A, B, C, M, N, K, st, ws, ws_elems);
     } else {
-        launch_gemm_t<false, false>(A, B, C, M, N, K, st);
+        launch_gemm_t<false, false>(A, B, C, M, N, K, st, ws, ws_elems);
+    }
+}
+
+// dst[n][k] = src[k][n] for a K x N bf16 matrix, through a 32 x 32 LDS tile.
+__global__ void transpose_bf16_kernel(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst, int K, int N) {
+    __shared__ uint16_t tile[32][33];
+    const int n0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 256 threads: 8 rows per pass
+    for (int r = ty; r < 32; r += 8) {
+        const int k = k0 + r, n = n0 + tx;
+        tile[r][tx] = (k < K && n < N) ? src[(int64_t)k * N + n] : (uint16_t)0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int n = n0 + r, k = k0 + tx;
+        if (n < N && k < K) { dst[(int64_t)n * K + k] = tile[tx][r]; }
     }
 }
 
@@ -264,8 +322,6 @@ __global__ void to_bf16_kernel(const float* __restrict__ x, uint16_t* __restrict
     if (i < n) { y[i] = f32_to_bf16(x[i]); }
 }
 
-unsigned ew_blocks(int64_t n) { return (unsigned)((n + EW_BLOCK - 1) / EW_BLOCK); }
-
 }  // namespace
 
 struct psim_mlp {
@@ -277,7 +333,10 @@ struct psim_mlp {
     std::vector<int64_t> w_off, b_off;  // offsets in the flat parameter vector
     int64_t n_params = 0;
     float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr, *mask = nullptr;
-    uint16_t* pb = nullptr;  // bf16 copy (weights used by the GEMMs; biases unused)
+    uint16_t* pb = nullptr;  // bf16 copy, layout of p (weights [in][out]; biases unused)
+    uint16_t* pbT = nullptr;  // bf16 weights transposed, [out][in] per layer (K-major for the forward)
+    float* ws = nullptr;  // partial products of weight gradients split over K
+    size_t ws_elems = 0;
     int64_t step = 0;
     // batch buffers
     int cap = 0;
@@ -353,9 +412,21 @@ int ensure_batch(psim_mlp* h, int rows) {
 }
 
 template <bool TA, bool TB>
-int gemm(psim_mlp* h, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K) {
-    launch_gemm(TA, TB, A, B, C, M, N, K, h->stream);
+int gemm(psim_mlp* h, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K, bool split = false) {
+    launch_gemm(TA, TB, A, B, C, M, N, K, h->stream, split ? h->ws : nullptr, split ? h->ws_elems : 0);
     MLP_CHECK(h, hipGetLastError());
+    return PSIM_OK;
+}
+
+// pbT from pb: every layer's weights transposed.
+int refresh_transposed(psim_mlp* h) {
+    for (size_t l = 0; l + 1 < h->dims.size(); ++l) {
+        const int K = h->dims[l], N = h->dims[l + 1];
+        const dim3 grid((unsigned)((N + 31) / 32), (unsigned)((K + 31) / 32));
+        hipLaunchKernelGGL(transpose_bf16_kernel, grid, dim3(256), 0, h->stream, h->pb + h->w_off[l],
+            h->pbT + h->w_off[l], K, N);
+        MLP_CHECK(h, hipGetLastError());
+    }
     return PSIM_OK;
 }
 
@@ -368,7 +439,8 @@ int forward(psim_mlp* h, int rows) {
     MLP_CHECK(h, hipGetLastError());
     for (int l = 0; l < L; ++l) {
         const int K = h->dims[l], N = h->dims[l + 1];
-        if ((rc = gemm<false, false>(h, h->a[l], h->pb + h->w_off[l], h->z[l], rows, N, K)) != 0) { return rc; }
+        // B = W^T stored [N][K]: both operands K-major
+        if ((rc = gemm<false, true>(h, h->a[l], h->pbT + h->w_off[l], h->z[l], rows, N, K)) != 0) { return rc; }
         hipLaunchKernelGGL(bias_act_kernel, dim3(ew_blocks((int64_t)rows * N)), dim3(EW_BLOCK), 0, h->stream, h->z[l],
             h->p + h->b_off[l], h->a[l + 1], (int64_t)rows, N, l + 1 < L ? 1 : 0);
         MLP_CHECK(h, hipGetLastError());
@@ -388,7 +460,7 @@ int backward(psim_mlp* h, int rows) {
     for (int l = L - 1; l >= 0; --l) {
         const int K = h->dims[l], N = h->dims[l + 1];
         // dW[K][N] = a_l^T [K][rows] x dz [rows][N]; db[N] = column sums of dz
-        if ((rc = gemm<true, false>(h, h->a[l], dz, h->g + h->w_off[l], K, N, rows)) != 0) { return rc; }
+        if ((rc = gemm<true, false>(h, h->a[l], dz, h->g + h->w_off[l], K, N, rows, true)) != 0) { return rc; }
         hipLaunchKernelGGL(bias_grad_kernel, dim3((unsigned)N), dim3(EW_BLOCK), 0, h->stream, dz, h->g + h->b_off[l],
             rows, N);
         MLP_CHECK(h, hipGetLastError());
@@ -456,7 +528,12 @@ int psim_mlp_create(psim_mlp** out, int device, int32_t n_in, int32_t hidden, in
     h->m2 = mlp_alloc<float>((size_t)off);
     h->mask = mlp_alloc<float>((size_t)off);
     h->pb = mlp_alloc<uint16_t>((size_t)off);
-    if (!h->p || !h->g || !h->m1 || !h->m2 || !h->mask || !h->pb) {
+    h->pbT = mlp_alloc<uint16_t>((size_t)off);
+    size_t widest = 0;
+    for (size_t l = 0; l + 1 < h->dims.size(); ++l) { widest = std::max(widest, (size_t)h->dims[l] * h->dims[l + 1]); }
+    h->ws_elems = (size_t)MAX_SPLITS * widest;
+    h->ws = mlp_alloc<float>(h->ws_elems);
+    if (!h->p || !h->g || !h->m1 || !h->m2 || !h->mask || !h->pb || !h->pbT || !h->ws) {
         psim_mlp_destroy(h);
         return PSIM_E_DEVICE;
     }
@@ -471,7 +548,8 @@ int psim_mlp_create(psim_mlp** out, int device, int32_t n_in, int32_t hidden, in
         || hipMemset(h->p, 0, sizeof(float) * (size_t)off) != hipSuccess
         || hipMemset(h->m1, 0, sizeof(float) * (size_t)off) != hipSuccess
         || hipMemset(h->m2, 0, sizeof(float) * (size_t)off) != hipSuccess
-        || hipMemset(h->pb, 0, sizeof(uint16_t) * (size_t)off) != hipSuccess) {
+        || hipMemset(h->pb, 0, sizeof(uint16_t) * (size_t)off) != hipSuccess
+        || hipMemset(h->pbT, 0, sizeof(uint16_t) * (size_t)off) != hipSuccess) {
         psim_mlp_destroy(h);
         return PSIM_E_DEVICE;
     }
@@ -484,7 +562,7 @@ void psim_mlp_destroy(psim_mlp* h) {
     (void)hipSetDevice(h->device);
     if (h->stream) { (void)hipStreamSynchronize(h->stream); }
     free_batch(h);
-    void* ps[] = { h->p, h->g, h->m1, h->m2, h->mask, h->pb };
+    void* ps[] = { h->p, h->g, h->m1, h->m2, h->mask, h->pb, h->pbT, h->ws };
     for (void* q : ps) {
         if (q) { (void)hipFree(q); }
     }
@@ -510,6 +588,8 @@ int psim_mlp_set_params(psim_mlp* h, const float* params) {
     hipLaunchKernelGGL(to_bf16_kernel, dim3(ew_blocks(h->n_params)), dim3(EW_BLOCK), 0, h->stream, h->p, h->pb,
         h->n_params);
     MLP_CHECK(h, hipGetLastError());
+    int rc;
+    if ((rc = refresh_transposed(h)) != 0) { return rc; }
     MLP_CHECK(h, hipMemsetAsync(h->m1, 0, sizeof(float) * (size_t)h->n_params, h->stream));
     MLP_CHECK(h, hipMemsetAsync(h->m2, 0, sizeof(float) * (size_t)h->n_params, h->stream));
     MLP_CHECK(h, hipStreamSynchronize(h->stream));
@@ -567,6 +647,7 @@ int psim_mlp_train_step(psim_mlp* h, const float* x, const float* t, int32_t row
     hipLaunchKernelGGL(adam_kernel, dim3(ew_blocks(h->n_params)), dim3(EW_BLOCK), 0, h->stream, h->p, h->g, h->m1, h->m2,
         h->mask, h->pb, h->n_params, lr, b1, b2, 1e-8f, c1, c2);
     MLP_CHECK(h, hipGetLastError());
+    if ((rc = refresh_transposed(h)) != 0) { return rc; }
     if (loss) { return read_loss(h, rows, loss); }
     MLP_CHECK(h, hipStreamSynchronize(h->stream));
     return PSIM_OK;
@@ -583,16 +664,17 @@ int psim_mlp_gemm_check(int device, int32_t ta, int32_t tb, int32_t M, int32_t N
     for (size_t i = 0; i < na; ++i) { ha[i] = f32_to_bf16(A[i]); }
     for (size_t i = 0; i < nb; ++i) { hb[i] = f32_to_bf16(B[i]); }
     uint16_t *da = nullptr, *db = nullptr;
-    float* dc = nullptr;
+    float *dc = nullptr, *ws = nullptr;
+    const size_t ws_elems = (size_t)MAX_SPLITS * nc;  // products split over K are checked too
     int rc = PSIM_OK;
     if (hipMalloc(&da, na * 2) != hipSuccess || hipMalloc(&db, nb * 2) != hipSuccess
-        || hipMalloc(&dc, nc * 4) != hipSuccess) {
+        || hipMalloc(&dc, nc * 4) != hipSuccess || hipMalloc(&ws, ws_elems * 4) != hipSuccess) {
         rc = PSIM_E_DEVICE;
     } else if (hipMemcpy(da, ha.data(), na * 2, hipMemcpyHostToDevice) != hipSuccess
                || hipMemcpy(db, hb.data(), nb * 2, hipMemcpyHostToDevice) != hipSuccess) {
         rc = PSIM_E_DEVICE;
     } else {
-        launch_gemm(ta != 0, tb != 0, da, db, dc, M, N, K, 0);
+        launch_gemm(ta != 0, tb != 0, da, db, dc, M, N, K, 0, ws, ws_elems);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess
             || hipMemcpy(C, dc, nc * 4, hipMemcpyDeviceToHost) != hipSuccess) {
             rc = PSIM_E_DEVICE;
@@ -601,31 +683,34 @@ int psim_mlp_gemm_check(int device, int32_t ta, int32_t tb, int32_t M, int32_t N
     if (da) { (void)hipFree(da); }
     if (db) { (void)hipFree(db); }
     if (dc) { (void)hipFree(dc); }
+    if (ws) { (void)hipFree(ws); }
     return rc;
 }
 
-// Average duration of one launch of the network's GEMM kernel for op(A)[M][K] x op(B)[K][N], from
-// HIP events around `reps` launches on a stream of its own (operands filled on the device).
+// Average duration of one GEMM of the network for op(A)[M][K] x op(B)[K][N] (a product split over
+// K includes its split reduce), from HIP events around `reps` products on a stream of its own
+// (operands filled on the device).
 int psim_mlp_time_gemm(int device, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, int32_t reps,
     float* ms_per_launch) {
     if (!ms_per_launch || M <= 0 || N <= 0 || K <= 0 || reps <= 0) { return PSIM_E_INVALID; }
     if (hipSetDevice(device) != hipSuccess) { return PSIM_E_DEVICE; }
     const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
     uint16_t *da = nullptr, *db = nullptr;
-    float *dc = nullptr, *src = nullptr;
+    float *dc = nullptr, *src = nullptr, *ws = nullptr;
+    const size_t ws_elems = (size_t)MAX_SPLITS * nc;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = PSIM_OK;
     const size_t nsrc = std::max(na, nb);
     if (hipMalloc(&da, na * 2) != hipSuccess || hipMalloc(&db, nb * 2) != hipSuccess
         || hipMalloc(&dc, nc * 4) != hipSuccess || hipMalloc(&src, nsrc * 4) != hipSuccess
-        || hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess
+        || hipMalloc(&ws, ws_elems * 4) != hipSuccess || hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess
         || hipEventCreate(&e1) != hipSuccess) {
         rc = PSIM_E_DEVICE;
     } else {
         std::vector<float> h(nsrc);
         for (size_t i = 0; i < nsrc; ++i) { h[i] = (float)((i * 2654435761u) % 1000) * 1e-3f - 0.5f; }
-        auto launch = [&] { launch_gemm(ta != 0, tb != 0, da, db, dc, M, N, K, st); };
+        auto launch = [&] { launch_gemm(ta != 0, tb != 0, da, db, dc, M, N, K, st, ws, ws_elems); };
         float ms = 0.f;
         if (hipMemcpy(src, h.data(), nsrc * 4, hipMemcpyHostToDevice) != hipSuccess) {
             rc = PSIM_E_DEVICE;
@@ -647,7 +732,7 @@ int psim_mlp_time_gemm(int device, int32_t ta, int32_t tb, int32_t M, int32_t N,
     if (e0) { (void)hipEventDestroy(e0); }
     if (e1) { (void)hipEventDestroy(e1); }
     if (st) { (void)hipStreamDestroy(st); }
-    void* bufs[] = { da, db, dc, src };
+    void* bufs[] = { da, db, dc, src, ws };
     for (void* q : bufs) {
         if (q) { (void)hipFree(q); }
     }

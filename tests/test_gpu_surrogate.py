@@ -24,12 +24,13 @@ def surrogate():
     return s
 
 
-@pytest.mark.parametrize("shape", [(70, 45, 100), (72, 40, 104), (4100, 4104, 64)],
-                         ids=["ragged", "vector-loads", "large-tiles"])
+@pytest.mark.parametrize("shape", [(70, 45, 100), (72, 40, 104), (4100, 4104, 64), (70, 45, 1100)],
+                         ids=["ragged", "vector-loads", "large-tiles", "split-k"])
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
 def test_gemm_matches_host_product(surrogate, ta, tb, shape):
     """Ragged shapes load element by element, shapes whose rows are multiples of 8 load 16-byte
-    vectors, and products of at least 1024 128 x 128 blocks use the large tile."""
+    vectors, products of at least 1024 128 x 128 blocks use the large tile, and a product of few
+    blocks with K >= 1024 is split over K and summed in order."""
     rng = np.random.default_rng(3 + 2 * ta + tb)
     M, N, K = shape
     a = rng.normal(size=(K, M) if ta else (M, K)).astype(np.float32)
@@ -81,10 +82,11 @@ def _reference_gradients(net, params, x, t):
     return float(loss.detach()), np.concatenate(flat)
 
 
-def test_gradients_match_cpu_reference(surrogate):
+@pytest.mark.parametrize("rows", [150, 2100])
+def test_gradients_match_cpu_reference(surrogate, rows):
+    """At 2100 rows the weight gradients (K = rows) are split over K."""
     net = surrogate.Surrogate(n_in=32, hidden=96, layers=3, seed=1)
     rng = np.random.default_rng(5)
-    rows = 150
     x = rng.normal(size=(rows, 32)).astype(np.float32)
     t = rng.normal(size=rows).astype(np.float32)
     params = net.params()
