@@ -73,3 +73,75 @@ def test_training_and_held_out_structures_are_disjoint():
     key = lambda p: (round(p["length"], 9), round(p["width"], 9), p["n_sensors"])  # noqa: E731
     assert not {key(p) for p in train} & {key(p) for p in held}
     assert all(abs(p["t_hot"] - p["t_cold"]) >= train_surrogate.MIN_WALL_DIFF for p in train + held)
+
+
+def _fake_relaxation(p, inits, steps, tau_steps):
+    """Per-step temperatures relaxing from `inits` (t_eq when None) to the linear profile with
+    time constant tau_steps, plus tiny deterministic noise."""
+    t_eq = 0.5 * (p["t_hot"] + p["t_cold"])
+    target = np.array(surrogate.linear_inits(p))
+    start = np.full(p["n_sensors"], t_eq) if inits is None else np.asarray(inits, float)
+    t = np.arange(steps)
+    noise = 1e-6 * np.sin(np.arange(p["n_sensors"])[:, None] * 0.7 + t[None, :] * 1.3)
+    return target[:, None] + (start - target)[:, None] * np.exp(-t / tau_steps)[None, :] + noise
+
+
+def test_evaluation_bookkeeping(monkeypatch):
+    """train_surrogate.evaluate on synthetic relaxations: a start at the settled field reaches
+    steady state at the earliest report (4 windows), a start at t_eq later; totals, ratios and
+    initial errors follow."""
+    params = train_surrogate.structures(train_surrogate.EVAL_SEED, 4)
+
+    def fake_runs(engine, ps, inits, phonons, steps, sim_ns, seed, reduce, batch=256, threads=16, progress=None):
+        return [reduce(p, _fake_relaxation(p, None if inits is None else inits[i], steps, 600.0))
+                for i, p in enumerate(ps)]
+
+    class PerfectNet:
+        """Predicts the linear profile, which the synthetic runs settle to."""
+
+        def predict(self, x):
+            return (1.0 - 2.0 * x[:, 8]).astype(np.float32)  # u of the linear profile at position s
+
+    monkeypatch.setattr(surrogate, "run_relaxations", fake_runs)
+    res = train_surrogate.evaluate(None, PerfectNet(), params, 1000, 5000, 25.0, 0, linear_count=2,
+                                   progress=lambda m: None, cost=False)
+    s = res["summary"]
+    row = s["criteria"][str(train_surrogate.HEADLINE_TOL)]
+    assert row["warm"]["reached"] == 4 and row["warm"]["total_ns"] == 4 * 2.0  # 400 steps of 5 ps each
+    assert row["cold"]["total_ns"] > row["warm"]["total_ns"]
+    assert row["cold_over_warm_total"] == row["cold"]["total_ns"] / row["warm"]["total_ns"]
+    assert s["initial_error_rmse_K"]["warm"] < 1e-3 < s["initial_error_rmse_K"]["cold"]
+    assert s["initial_error_rmse_K"]["linear"] < 1e-3
+    assert len(res["rows"]) == 4 and set(res["rows"][0]) >= {"cold_ttss", "warm_ttss", "linear_ttss"}
+
+
+def test_cut_runs_keep_the_energy_per_phonon(monkeypatch):
+    """gpu_seconds_to_steady_state groups runs of equal length into one batch and scales each
+    budget so that the cut run's energy per phonon equals the full run's."""
+    import psim_amd
+
+    params = train_surrogate.structures(train_surrogate.EVAL_SEED, 5)
+    batches = []
+
+    def fake_run_batch(engine, models, seed):
+        batches.append([(m.num_record, m.prepare(0)) for m in models])
+        return {"phonon_steps": 7}
+
+    class Clock:
+        def last_timing(self):
+            return {"run_ms": 250.0}
+
+    monkeypatch.setattr(psim_amd, "run_batch", fake_run_batch)
+    out = surrogate.gpu_seconds_to_steady_state(Clock(), params, None, [400, 1800, None, 400, 2500], 8_000_000,
+                                                5000, 25.0, seed=0)
+    assert [len(b) for b in batches] == [2, 1, 1, 1]  # 400 x2, 1800, 2500, 5000 (the None)
+    assert [b[0][0] for b in batches] == [400, 1800, 2500, 5000]
+    assert out == {"gpu_s": 1.0, "phonon_steps": 28, "runs": 4}
+    for i, b in zip([0, 1, 4, 2], [batches[0][0], batches[1][0], batches[2][0], batches[3][0]]):
+        full = psim_amd.Model(configs.geometry_from_params(dict(params[i], phonons=8_000_000), 5000, 25.0,
+                                                          sim_type=1, step_interval=100))
+        eff_full = full.prepare(0)[0]
+        n_steps, (eff_cut, total_cut, _) = b
+        # the cut budget is rounded to whole phonons
+        assert abs(eff_cut / eff_full - 1.0) < 1e-5
+        assert abs(total_cut - 8_000_000 * n_steps / 5000) <= 2
