@@ -284,7 +284,11 @@ enum : int { S_FETCH = 0, S_TTS = 1, S_NI = 2, S_COLL = 3, S_SCAT = 4, S_EXIT = 
 // Each lane still performs exactly the reference's sequence of operations for its phonon, so
 // trajectories are unchanged.
 // TALLY: the tally path of this launch (TALLY_NONE, TALLY_LDS, TALLY_GLOBAL).
-template <int TALLY>
+// MULTI: some source has a stream index or id base (a batch of models).  Otherwise a phonon's
+// id within its stream is its id and the stream word is the bare tag, so the single-model
+// variant neither loads nor stores the per-phonon stream id and keeps that word a constant;
+// both variants draw the same numbers for such a model.
+template <int TALLY, bool MULTI>
 __global__ void __launch_bounds__(BLOCK)
 __attribute__((amdgpu_waves_per_eu(TALLY == TALLY_NONE ? PSIM_UNTALLIED_WAVES_PER_EU : PSIM_TRANSPORT_WAVES_PER_EU)))
 transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const uint64_t* __restrict__ seed_p,
@@ -452,7 +456,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                     slot = started ? L.id[item] : lists.born_order[b0 + (item - n_live)];
                     const int64_t j = started ? item : (int64_t)slot;
                     const uint32_t meta = (started ? L.meta : pool.meta)[j];
-                    const uint32_t sid = (started ? L.sid : pool.sid)[j];
+                    const uint32_t sid = MULTI ? (started ? L.sid : pool.sid)[j] : slot;
                     p.cell = (started ? L.cell : pool.cell)[j];
                     cur_sensor = (started ? L.sensor : pool.sensor)[j];
                     p.px = (started ? L.px : pool.px)[j];
@@ -475,7 +479,8 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                     ttm = 0.;
                     events = 0;
                     // the stream keeps (stream index, id within it) in its counter words c3 and c2
-                    psim_stream_init(&st, seed, psim_stream_tag(PSIM_TAG_TRANSPORT, meta >> META_STREAM_SHIFT), sid,
+                    psim_stream_init(&st, seed,
+                        MULTI ? psim_stream_tag(PSIM_TAG_TRANSPORT, meta >> META_STREAM_SHIFT) : PSIM_TAG_TRANSPORT, sid,
                         (uint32_t)p.step);
                     begin_event();
                 }
@@ -571,9 +576,9 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                 O.cell[k] = p.cell;
                 O.sensor[k] = cur_sensor;
                 O.meta[k] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
-                            | META_STARTED | ((st.c3 >> 8) << META_STREAM_SHIFT);
+                            | META_STARTED | (MULTI ? (st.c3 >> 8) << META_STREAM_SHIFT : 0u);
                 O.id[k] = slot;
-                O.sid[k] = st.c2;
+                if (MULTI) { O.sid[k] = st.c2; }
                 state = S_FETCH;
             }
         }
@@ -671,7 +676,7 @@ __global__ void __launch_bounds__(1024) cell_scan_kernel(uint32_t* __restrict__ 
 }
 
 __global__ void __launch_bounds__(SORT_BLOCK) cell_scatter_kernel(Live src, Live dst,
-    const uint32_t* __restrict__ n_ptr, int shift, int n_keys, uint32_t* __restrict__ cursor) {
+    const uint32_t* __restrict__ n_ptr, int shift, int n_keys, uint32_t* __restrict__ cursor, int copy_sid) {
     __shared__ uint32_t cnt[SORT_KEYS];
     __shared__ uint32_t base[SORT_KEYS];
     for (int k = threadIdx.x; k < n_keys; k += SORT_BLOCK) { cnt[k] = 0; }
@@ -708,7 +713,7 @@ __global__ void __launch_bounds__(SORT_BLOCK) cell_scatter_kernel(Live src, Live
         dst.sensor[k] = src.sensor[i];
         dst.meta[k] = src.meta[i];
         dst.id[k] = src.id[i];
-        dst.sid[k] = src.sid[i];
+        if (copy_sid) { dst.sid[k] = src.sid[i]; }
     }
 }
 
@@ -771,6 +776,7 @@ struct psim_gpu_ctx {
     std::string err;
     // model
     bool uploaded = false;
+    bool multi_stream = false;  // some source has a stream index or id base (a batch)
     psim_run_params params{};
     Model m{};
     std::vector<void*> model_allocs;
@@ -1086,6 +1092,31 @@ int zero_fill(psim_gpu_ctx* c, void* p, size_t bytes) {
     return PSIM_OK;
 }
 
+// One transport launch of the variant for this tally path and stream layout.
+void launch_transport(psim_gpu_ctx* c, int tally, int grid, size_t lds, const Lists& lists, int32_t k0, int32_t k1,
+    const Tally& t, const Fate& fate) {
+    const dim3 g((unsigned)grid), b(BLOCK);
+#define PSIM_TRANSPORT(T, M)                                                                                 \
+    hipLaunchKernelGGL((transport_kernel<T, M>), g, b, lds, c->stream, c->m, c->pool, lists, k0, k1, c->d_seed, t, \
+        c->d_counters, fate)
+    if (c->multi_stream) {
+        if (tally == TALLY_NONE) {
+            PSIM_TRANSPORT(TALLY_NONE, true);
+        } else if (tally == TALLY_GLOBAL) {
+            PSIM_TRANSPORT(TALLY_GLOBAL, true);
+        } else {
+            PSIM_TRANSPORT(TALLY_LDS, true);
+        }
+    } else if (tally == TALLY_NONE) {
+        PSIM_TRANSPORT(TALLY_NONE, false);
+    } else if (tally == TALLY_GLOBAL) {
+        PSIM_TRANSPORT(TALLY_GLOBAL, false);
+    } else {
+        PSIM_TRANSPORT(TALLY_LDS, false);
+    }
+#undef PSIM_TRANSPORT
+}
+
 // Every launch of one run, on c->stream.  `eager`: also times each transport window with events
 // (and prints a progress line per window when PSIM_PROGRESS is set); otherwise the sequence is
 // free of host synchronisation, so it can be captured into a graph.
@@ -1128,16 +1159,8 @@ int enqueue_run(psim_gpu_ctx* c, const RunPlan& plan, bool eager) {
         Tally t{ c->d_pcnt, c->d_pfx, c->d_pfy, c->d_counts, c->d_flux, rec ? 1 : 0, global_tally ? 1 : 0, W };
         const size_t lds = (rec && !global_tally) ? (size_t)WS * 20 : 0;
         if (eager) { HIP_CHECK(c, hipEventRecord(c->ev[1], c->stream)); }
-        if (!rec) {  // no step of this window is recorded
-            hipLaunchKernelGGL(transport_kernel<TALLY_NONE>, dim3(grid), dim3(BLOCK), 0, c->stream, c->m, c->pool,
-                lists, (int32_t)k0, (int32_t)k1, c->d_seed, t, c->d_counters, fate);
-        } else if (global_tally) {
-            hipLaunchKernelGGL(transport_kernel<TALLY_GLOBAL>, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool,
-                lists, (int32_t)k0, (int32_t)k1, c->d_seed, t, c->d_counters, fate);
-        } else {
-            hipLaunchKernelGGL(transport_kernel<TALLY_LDS>, dim3(grid), dim3(BLOCK), lds, c->stream, c->m, c->pool,
-                lists, (int32_t)k0, (int32_t)k1, c->d_seed, t, c->d_counters, fate);
-        }
+        launch_transport(c, !rec ? TALLY_NONE : global_tally ? TALLY_GLOBAL : TALLY_LDS, grid, !rec ? 0 : lds, lists,
+            (int32_t)k0, (int32_t)k1, t, fate);
         HIP_CHECK(c, hipGetLastError());
         if (eager) { HIP_CHECK(c, hipEventRecord(c->ev[2], c->stream)); }
         if (rec && !global_tally) {
@@ -1164,7 +1187,7 @@ int enqueue_run(psim_gpu_ctx* c, const RunPlan& plan, bool eager) {
                 c->d_sort_keys + SORT_KEYS);
             HIP_CHECK(c, hipGetLastError());
             hipLaunchKernelGGL(cell_scatter_kernel, dim3(sgrid), dim3(SORT_BLOCK), 0, c->stream, c->live[1], c->live[0],
-                n_ptr, shift, n_keys, c->d_sort_keys);
+                n_ptr, shift, n_keys, c->d_sort_keys, c->multi_stream ? 1 : 0);
             HIP_CHECK(c, hipGetLastError());
         }
         if (!eager) { continue; }
@@ -1373,8 +1396,10 @@ int psim_gpu_upload(psim_gpu_ctx* c, const psim_geom* g, const psim_tables* t, c
         }
     }
     int64_t expect = 0;
+    bool multi = false;
     for (int i = 0; i < s->n_sources; ++i) {
         const psim_source& src = s->sources[i];
+        multi = multi || src.stream != 0 || src.id_base != 0;
         if (src.cell < 0 || src.cell >= g->n_cells || src.table < 0 || src.table >= t->n_tables || src.count < 0
             || src.offset != expect || (src.kind != 0 && src.kind != 1) || src.stream < 0 || src.stream > 0xffff
             || src.id_base < 0 || src.id_base > src.offset
@@ -1458,6 +1483,7 @@ int psim_gpu_upload(psim_gpu_ctx* c, const psim_geom* g, const psim_tables* t, c
     c->n_sensors = g->n_sensors;
     c->n_cells = g->n_cells;
     c->uploaded = true;
+    c->multi_stream = multi;
     c->gen++;
     return PSIM_OK;
 }
