@@ -35,6 +35,19 @@ SILICON_HOLLAND = {
 }
 
 
+# Germanium as the reference ships it (psim_python/psim/pre_builts.py:25-28, "not used" there).
+GERMANIUM = {
+    "name": "Germanium",
+    "d_data": {
+        "la_data": [-1.50e-7, 5.63e3, 0.0],
+        "max_freq_la": 4.45236386e13,
+        "ta_data": [-1.13e-7, 2.60e3, 0.0],
+        "max_freq_ta": 1.4937724175e13,
+    },
+    "r_data": {"b_l": 2.3e-24, "b_tn": 30.0e-13, "b_tu": 1.5e-18, "b_i": 0.0, "w": 1.23e13},
+}
+
+
 class ModelBuilder:
     """Accumulates a psim model and serialises it to the JSON schema."""
 
@@ -101,6 +114,24 @@ def linear_bar(num_sensors: int = 10, length: float = 1000.0, width: float = 100
     b.add_surface((0.0, 0.0), (0.0, width), t_hot)
     b.add_surface((length, 0.0), (length, width), t_cold)
     return b
+
+
+def si_ge_bar(num_sensors: int = 10, num_phonons: int = 20000, num_measurements: int = 1000, sim_time: float = 5.0,
+              spec: float = 0.3, t_eq: float = 300.0, sim_type: int = 0, step_interval: int = 0,
+              phasor_sim: bool = False) -> dict:
+    """The linear bar with its second half germanium: every transition between the halves joins
+    two materials, where a phonon above the other side's maximum frequency of its branch is
+    scattered back diffusely (TransitionSurface::handlePhonon, psim/src/surface.cpp:71-109)."""
+    b = ModelBuilder(num_measurements, sim_time, num_phonons, t_eq, sim_type, step_interval, phasor_sim=phasor_sim)
+    b.materials.append(copy.deepcopy(GERMANIUM))
+    length, width = 100.0 * num_sensors, 100.0
+    dx = length / num_sensors
+    for i in range(num_sensors):
+        sid = b.add_sensor(300.0, "Silicon" if i < num_sensors // 2 else "Germanium")
+        b.add_rectangle((i * dx, 0.0), ((i + 1) * dx, width), sid, spec)
+    b.add_surface((0.0, 0.0), (0.0, width), 310.0)
+    b.add_surface((length, 0.0), (length, width), 290.0)
+    return b.to_dict()
 
 
 def linear_ss(num_phonons: int = 100_000) -> dict:

@@ -1,13 +1,16 @@
 """Write the reference fixtures under tests/golden/ (run in the build container only).
 
 Builds the reference psim from /root/reference with `make -C oracle ref` (its
-own sources plus oracle/ref_driver.cpp) and runs it on three workloads:
+own sources plus oracle/ref_driver.cpp) and runs it on these workloads:
 
   linear_ss     the 1000x100 nm bar, deviational steady state, 1e5 phonons, 2000 steps
   lshape_r      the conforming L, full mode, reduced to 1e5 phonons (5000 steps)
   film_r        the transient film, reduced to 1e5 phonons and 2000 steps over 2 ns
   linear_k      the linear_ss bar at 1e6 phonons over 48 runs (statistics only, with every
                 run's fluxes): pins the effective conductivity to a few tenths of a percent
+  si_ge         the bar with its second half germanium (configs.si_ge_bar), 1e5 phonons,
+                2000 steps: transitions between two materials (statistics only)
+  phasor        the bar in phasor mode (statistics only)
 
 For each workload it stores
   ref_<name>.json  the model, and over N independent reference runs (the
@@ -52,7 +55,15 @@ def workloads():
         "film_r": (configs.film_transient(num_phonons=100_000, num_measurements=2000, sim_time=2.0),
                    [300.0, 320.0]),
         "linear_k": (configs.linear_ss(num_phonons=1_000_000), None),
+        "si_ge": (configs.si_ge_bar(num_phonons=100_000, num_measurements=2000, sim_time=10.0), None),
+        "phasor": (_phasor_bar(), None),
     }
+
+
+def _phasor_bar() -> dict:
+    m = configs.linear_bar(num_phonons=100_000, num_measurements=2000, sim_time=10.0, spec=0.3).to_dict()
+    m["settings"]["phasor_sim"] = True
+    return m
 
 
 RUNS = {"linear_k": 48}  # runs per workload where the default is not enough

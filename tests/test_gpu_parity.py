@@ -32,6 +32,11 @@ CASES = {
     "lshape_full": configs.lshape_full(num_phonons=20000, num_measurements=1000),
     "film_transient": configs.film_transient(num_phonons=20000, num_measurements=1000, sim_time=1.0,
                                              step_interval=50),
+    # silicon | germanium: transitions between two materials
+    "bar_si_ge": configs.si_ge_bar(),
+    "bar_si_ge_full_mode": configs.si_ge_bar(t_eq=0.0),
+    # phasor mode: fixed speed and branch, no scattering
+    "bar_phasor": configs.si_ge_bar(phasor_sim=True),
 }
 
 

@@ -297,3 +297,25 @@ def test_oracle_statistics_match_reference(name):
     for key in ["t_steady", "x_flux", "y_flux"]:
         z = _three_sigma(fix, ours, key)
         assert np.all(z <= 3.0), (key, np.round(z, 2))
+
+
+STATS_ONLY = ["si_ge", "phasor"]
+
+
+@pytest.mark.parametrize("name", STATS_ONLY)
+def test_oracle_statistics_match_reference_on_interfaces_and_phasors(name):
+    """Paths the three workloads above do not take: transitions between two materials (a
+    phonon above the neighbour's maximum frequency of its branch scatters back diffusely,
+    surface.cpp:71-109), and phasor mode (fixed speed and branch, no scattering,
+    phononBuilder.cpp:42-49).  16 oracle seeds against 16 reference runs, per sensor."""
+    with open(os.path.join(GOLDEN, f"ref_{name}.json")) as f:
+        fix = json.load(f)
+    ours = []
+    for seed in range(fix["runs"]):
+        m = oracle.Model(fix["model"])
+        r = m.run(seed, nthreads=8)
+        out, _, _ = m.interpret(r.counts, r.flux)
+        ours.append({"t_steady": out[:, 0], "x_flux": out[:, 2], "y_flux": out[:, 4]})
+    for key in ["t_steady", "x_flux", "y_flux"]:
+        z = _three_sigma(fix, ours, key)
+        assert np.all(z <= 3.0), (key, np.round(z, 2))
