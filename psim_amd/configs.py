@@ -116,6 +116,32 @@ def linear_bar(num_sensors: int = 10, length: float = 1000.0, width: float = 100
     return b
 
 
+def needle_bar(count: int = 6, base: float = 2.0, length: float = 400.0, spec: float = 1.0,
+               num_phonons: int = 20000, num_measurements: int = 1000, sim_time: float = 5.0) -> dict:
+    """The 1000x100 nm bar (walls 310 / 290 K, deviational about 300 K) with `count` specular
+    needles on its top edge, each a triangle of `base` nm on a whole top edge of its own column
+    of the bar and `length` nm high.  A phonon running into a needle's tip meets its two walls
+    more than MAX_COLLISIONS = 100 times in one drift, and the reference moves it to a random
+    point of the cell (ModelSimulator::handleImpacts, psim/src/modelSimulator.cpp:215-218)."""
+    b = ModelBuilder(num_measurements, sim_time, num_phonons, 300.0)
+    width = 100.0
+    xs = [1000.0 * (k + 1) / (count + 1) for k in range(count)]
+    cuts = [0.0]
+    for x in xs:
+        cuts += [x - base / 2, x + base / 2]
+    cuts.append(1000.0)
+    sensors = []
+    for i in range(len(cuts) - 1):
+        sid = b.add_sensor(300.0)
+        sensors.append(sid)
+        b.add_rectangle((cuts[i], 0.0), (cuts[i + 1], width), sid, spec)
+    for k, x in enumerate(xs):
+        b.add_triangle((x - base / 2, width), (x + base / 2, width), (x, width + length), sensors[2 * k + 1], spec)
+    b.add_surface((0.0, 0.0), (0.0, width), 310.0)
+    b.add_surface((1000.0, 0.0), (1000.0, width), 290.0)
+    return b.to_dict()
+
+
 def si_ge_bar(num_sensors: int = 10, num_phonons: int = 20000, num_measurements: int = 1000, sim_time: float = 5.0,
               spec: float = 0.3, t_eq: float = 300.0, sim_type: int = 0, step_interval: int = 0,
               phasor_sim: bool = False) -> dict:

@@ -11,6 +11,8 @@ own sources plus oracle/ref_driver.cpp) and runs it on these workloads:
   si_ge         the bar with its second half germanium (configs.si_ge_bar), 1e5 phonons,
                 2000 steps: transitions between two materials (statistics only)
   phasor        the bar in phasor mode (statistics only)
+  needles       the bar with six specular needles (configs.needle_bar): acute corners where
+                stuck phonons are relocated (statistics only)
 
 For each workload it stores
   ref_<name>.json  the model, and over N independent reference runs (the
@@ -57,6 +59,7 @@ def workloads():
         "linear_k": (configs.linear_ss(num_phonons=1_000_000), None),
         "si_ge": (configs.si_ge_bar(num_phonons=100_000, num_measurements=2000, sim_time=10.0), None),
         "phasor": (_phasor_bar(), None),
+        "needles": (configs.needle_bar(num_phonons=100_000, num_measurements=2000, sim_time=10.0), None),
     }
 
 

@@ -37,6 +37,8 @@ CASES = {
     "bar_si_ge_full_mode": configs.si_ge_bar(t_eq=0.0),
     # phasor mode: fixed speed and branch, no scattering
     "bar_phasor": configs.si_ge_bar(phasor_sim=True),
+    # specular needles: phonons stuck in a tip are relocated after 100 collisions in one drift
+    "bar_needles_relocation": configs.needle_bar(),
 }
 
 
@@ -67,6 +69,8 @@ def test_trajectories_match_oracle(engine, name):
     o = ref.run(seed, nthreads=8, fate=True)
     assert total == ref.total_phonons and eff == ref.eff_energy
     assert res.counters == o.counters
+    if name == "bar_needles_relocation":
+        assert o.counters["relocations"] > 0  # the stuck-phonon guard was exercised
     np.testing.assert_array_equal(g_cause, o.fate_cause)
     np.testing.assert_array_equal(g_step, o.fate_step)
     np.testing.assert_array_equal(g_xy, o.fate_xy)

@@ -299,15 +299,16 @@ def test_oracle_statistics_match_reference(name):
         assert np.all(z <= 3.0), (key, np.round(z, 2))
 
 
-STATS_ONLY = ["si_ge", "phasor"]
+STATS_ONLY = ["si_ge", "phasor", "needles"]
 
 
 @pytest.mark.parametrize("name", STATS_ONLY)
 def test_oracle_statistics_match_reference_on_interfaces_and_phasors(name):
     """Paths the three workloads above do not take: transitions between two materials (a
     phonon above the neighbour's maximum frequency of its branch scatters back diffusely,
-    surface.cpp:71-109), and phasor mode (fixed speed and branch, no scattering,
-    phononBuilder.cpp:42-49).  16 oracle seeds against 16 reference runs, per sensor."""
+    surface.cpp:71-109), phasor mode (fixed speed and branch, no scattering,
+    phononBuilder.cpp:42-49), and acute specular needles where stuck phonons are relocated
+    (modelSimulator.cpp:215-218).  16 oracle seeds against 16 reference runs, per sensor."""
     with open(os.path.join(GOLDEN, f"ref_{name}.json")) as f:
         fix = json.load(f)
     ours = []
