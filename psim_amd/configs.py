@@ -195,20 +195,22 @@ def lshape_full(num_phonons: int = 10_000_000, num_measurements: int = 5000) -> 
 
 
 def film_transient(num_phonons: int = 100_000_000, num_measurements: int = 20000, sim_time: float = 20.0,
-                   burst_temp: float = 320.0, burst_duration: float = 0.1, step_interval: int = 100) -> dict:
+                   burst_temp: float = 320.0, burst_duration: float = 0.1, step_interval: int = 100,
+                   burst_start: float = 0.0) -> dict:
     """1500x150 nm film in 30 sensors of 50x150 nm at 300 K.  The heat burst is
-    a set of transient emitting surfaces at 320 K active for [0, 0.1] ns on the
-    outer edges of sensors 0-2; the far end (x = 1500) is an absorbing 300 K
-    wall.  Transient simulations must be deviational (psim/src/model.cpp:58-60)."""
+    a set of transient emitting surfaces at 320 K active for [burst_start, burst_start +
+    burst_duration] ns (default [0, 0.1]) on the outer edges of sensors 0-2; outside that window
+    they reflect like plain walls.  The far end (x = 1500) is an absorbing 300 K wall.  Transient
+    simulations must be deviational (psim/src/model.cpp:58-60)."""
     b = ModelBuilder(num_measurements, sim_time, num_phonons, 300.0, sim_type=2, step_interval=step_interval)
     seg, w = 50.0, 150.0
     for i in range(30):
         sid = b.add_sensor(300.0)
         b.add_rectangle((i * seg, 0.0), ((i + 1) * seg, w), sid, 0.0)
-    b.add_surface((0.0, 0.0), (0.0, w), burst_temp, burst_duration, 0.0)
+    b.add_surface((0.0, 0.0), (0.0, w), burst_temp, burst_duration, burst_start)
     for i in range(3):
-        b.add_surface((i * seg, 0.0), ((i + 1) * seg, 0.0), burst_temp, burst_duration, 0.0)
-        b.add_surface((i * seg, w), ((i + 1) * seg, w), burst_temp, burst_duration, 0.0)
+        b.add_surface((i * seg, 0.0), ((i + 1) * seg, 0.0), burst_temp, burst_duration, burst_start)
+        b.add_surface((i * seg, w), ((i + 1) * seg, w), burst_temp, burst_duration, burst_start)
     b.add_surface((1500.0, 0.0), (1500.0, w), 300.0)
     return b.to_dict()
 

@@ -39,6 +39,9 @@ CASES = {
     "bar_phasor": configs.si_ge_bar(phasor_sim=True),
     # specular needles: phonons stuck in a tip are relocated after 100 collisions in one drift
     "bar_needles_relocation": configs.needle_bar(),
+    # a burst that starts at 0.3 ns: before it, its surfaces reflect like plain walls
+    "film_late_burst": configs.film_transient(num_phonons=20000, num_measurements=1000, sim_time=1.0,
+                                              step_interval=50, burst_start=0.3, burst_duration=0.2),
 }
 
 
@@ -106,3 +109,21 @@ def test_launch_window_does_not_change_results(engine, window):
     for a, b in zip(fate, base_fate):
         np.testing.assert_array_equal(a, b)
     assert _flux_close(res.raw_flux, base.raw_flux)
+
+
+def test_each_run_of_a_model_draws_from_its_own_seed(engine):
+    """A model with num_runs = 2 (Model::runSimulation's loop, model.cpp:141-182): every run
+    starts from reset controllers, and run r draws from seed + r * 0x9E3779B97F4A7C15, so it
+    equals the oracle's single run with that seed."""
+    cfg = configs.with_settings(_bar(0.3, n=10000), num_runs=2)
+    print("case two runs seed 11", flush=True)
+    model = psim_amd.Model(cfg)
+    engine.set_fate(False)
+    model.run(engine, 11)
+    for r in range(2):
+        res = model.results(r)
+        o = oracle.Model(cfg).run((11 + r * 0x9E3779B97F4A7C15) % 2**64, nthreads=8)
+        assert res.counters == o.counters
+        np.testing.assert_array_equal(res.counts, o.counts)
+        assert _flux_close(res.raw_flux, o.flux)
+    assert not np.array_equal(model.results(0).counts, model.results(1).counts)
