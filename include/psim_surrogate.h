@@ -33,7 +33,8 @@ int64_t psim_mlp_num_params(const psim_mlp* mlp);
 int psim_mlp_set_params(psim_mlp* mlp, const float* params);
 int psim_mlp_get_params(psim_mlp* mlp, float* params);
 
-/* y[r] = network(x[r][0 .. n_in)) for r < rows. */
+/* y[r] = network(x[r][0 .. n_in)) for r < rows.  Calls taking rows accept 1 .. 4194240 (65535
+ * blocks of 64 rows); more is rejected with PSIM_E_INVALID and a message. */
 int psim_mlp_forward(psim_mlp* mlp, const float* x, int32_t rows, float* y);
 
 /* Mean squared error against t and its gradient with respect to every parameter (FP32). */

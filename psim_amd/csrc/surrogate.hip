@@ -24,6 +24,8 @@ namespace {
 
 constexpr int GEMM_THREADS = 256;  // 4 waves
 constexpr int BATCH_ROUND = 64;  // batch buffers are sized in multiples of this many rows
+// Rows of one call: GEMM grids put row blocks of 64 on grid.y, which holds at most 65535 blocks.
+constexpr int MAX_ROWS = 65535 * 64;
 constexpr int PAD_OUT = 16;  // output-layer columns (1 used)
 constexpr int EW_BLOCK = 256;
 
@@ -607,6 +609,7 @@ int psim_mlp_get_params(psim_mlp* h, float* params) {
 
 int psim_mlp_forward(psim_mlp* h, const float* x, int32_t rows, float* y) {
     if (!h || !x || !y || rows <= 0) { return PSIM_E_INVALID; }
+    if (rows > MAX_ROWS) { return mlp_fail(h, PSIM_E_INVALID, "more than 4194240 rows in one call"); }
     MLP_CHECK(h, hipSetDevice(h->device));
     int rc;
     if ((rc = load_batch(h, x, nullptr, rows)) != 0) { return rc; }
@@ -621,6 +624,7 @@ int psim_mlp_forward(psim_mlp* h, const float* x, int32_t rows, float* y) {
 
 int psim_mlp_gradients(psim_mlp* h, const float* x, const float* t, int32_t rows, float* grad, float* loss) {
     if (!h || !x || !t || rows <= 0) { return PSIM_E_INVALID; }
+    if (rows > MAX_ROWS) { return mlp_fail(h, PSIM_E_INVALID, "more than 4194240 rows in one call"); }
     MLP_CHECK(h, hipSetDevice(h->device));
     int rc;
     if ((rc = load_batch(h, x, t, rows)) != 0) { return rc; }
@@ -636,6 +640,7 @@ int psim_mlp_gradients(psim_mlp* h, const float* x, const float* t, int32_t rows
 
 int psim_mlp_train_step(psim_mlp* h, const float* x, const float* t, int32_t rows, float lr, float* loss) {
     if (!h || !x || !t || rows <= 0 || !(lr > 0.f)) { return PSIM_E_INVALID; }
+    if (rows > MAX_ROWS) { return mlp_fail(h, PSIM_E_INVALID, "more than 4194240 rows in one call"); }
     MLP_CHECK(h, hipSetDevice(h->device));
     int rc;
     if ((rc = load_batch(h, x, t, rows)) != 0) { return rc; }
@@ -657,7 +662,7 @@ int psim_mlp_train_step(psim_mlp* h, const float* x, const float* t, int32_t row
 // lane layout and the transposed loads can be checked against a host product.
 int psim_mlp_gemm_check(int device, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const float* A,
     const float* B, float* C) {
-    if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0) { return PSIM_E_INVALID; }
+    if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0 || M > MAX_ROWS) { return PSIM_E_INVALID; }
     if (hipSetDevice(device) != hipSuccess) { return PSIM_E_DEVICE; }
     const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
     std::vector<uint16_t> ha(na), hb(nb);
@@ -692,7 +697,7 @@ int psim_mlp_gemm_check(int device, int32_t ta, int32_t tb, int32_t M, int32_t N
 // (operands filled on the device).
 int psim_mlp_time_gemm(int device, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, int32_t reps,
     float* ms_per_launch) {
-    if (!ms_per_launch || M <= 0 || N <= 0 || K <= 0 || reps <= 0) { return PSIM_E_INVALID; }
+    if (!ms_per_launch || M <= 0 || N <= 0 || K <= 0 || reps <= 0 || M > MAX_ROWS) { return PSIM_E_INVALID; }
     if (hipSetDevice(device) != hipSuccess) { return PSIM_E_DEVICE; }
     const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
     uint16_t *da = nullptr, *db = nullptr;

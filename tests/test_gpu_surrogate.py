@@ -147,3 +147,16 @@ def test_training_fits_a_smooth_function(surrogate):
     other = surrogate.Surrogate(seed=99)
     other.set_params(p)
     np.testing.assert_array_equal(other.predict(x[:512]), y)
+
+
+def test_too_many_rows_are_rejected(surrogate):
+    """Row counts beyond the GEMM grid's reach (65535 blocks of 64 rows) fail with a message
+    before any buffer is read."""
+    net = surrogate.Surrogate(n_in=32, hidden=64, layers=1, seed=0)
+    x = np.zeros((4, 32), np.float32)
+    y = np.zeros(4, np.float32)
+    L = net._L
+    rc = L.psim_mlp_forward(net._h, surrogate._fp(x), 65535 * 64 + 1, surrogate._fp(y))
+    assert rc != 0
+    assert "rows" in L.psim_mlp_last_error(net._h).decode()
+    np.testing.assert_array_equal(net.predict(x), net.predict(x))  # the handle still works
