@@ -430,6 +430,11 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    # tally atomics: every tallied phonon-step adds its count (int32) and two flux components (FP64)
+    # with LDS atomics (or HBM atomics when the window tallies do not fit LDS)
+    adds = 3 * counters["tallies"]
+    line["tally_atomics"] = {"adds_per_run": adds,
+                             "adds_per_s_of_transport_time": adds / (kernel_ms / 1e3 / args.steps)}
     # HBM traffic of the transport kernel per launch, measured by rocprofv3 --pmc FETCH_SIZE and
     # WRITE_SIZE (scaled by the copy-kernel calibration) on this workload; see profiles/.
     traffic_file = os.path.join(ROOT, "profiles", "transport_kernel_lshape_1e7_traffic.json")
