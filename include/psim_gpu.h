@@ -214,6 +214,15 @@ int psim_gpu_set_window(psim_gpu_ctx* ctx, int32_t steps_per_launch);
 /* Engine tuning: transport blocks per compute unit (default 8). */
 int psim_gpu_set_grid(psim_gpu_ctx* ctx, int32_t blocks_per_cu);
 
+/* Engine tuning: where recorded steps are tallied.  AUTO chooses by model size; LDS keeps
+ * per-block window partials in LDS (used whenever one step of tallies fits); HBM adds into the
+ * tallies with FP64 atomics in one launch.  Integer counts are the same either way; flux sums
+ * differ only in summation order.  No counterpart in the reference. */
+#define PSIM_TALLY_AUTO 0
+#define PSIM_TALLY_LDS 1
+#define PSIM_TALLY_HBM 2
+int psim_gpu_set_tally_mode(psim_gpu_ctx* ctx, int32_t mode);
+
 /* psim_gpu_run replayed from a HIP graph of the whole run (emission, sorts, every transport
  * window and reduction), captured on the first call and again after anything that changes the
  * launch sequence (upload, window, grid, fate recording).  The seed is data, so one graph serves

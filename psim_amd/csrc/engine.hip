@@ -40,6 +40,11 @@ constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
 constexpr int LDS_TALLY_BYTES = 48 * 1024;  // budget for one block's [W][S] window tally
 constexpr int GLOBAL_TALLY_STEPS = 8;  // fewer LDS-tallied steps per window than this: global atomics
+// Fewer phonons per sensor than this: global atomics too.  Small runs lose more to the many short
+// LDS windows (launches, reductions, sorts) than HBM atomics lose to contention, large ones the
+// reverse (measured at 1e4 and 1e5 phonons per sensor: 1.8-3x faster in HBM below, 2.7x slower
+// above).
+constexpr int64_t LDS_MIN_PHONONS_PER_SENSOR = 30000;
 constexpr uint32_t QUEUE_CLAIM = 32;  // work-queue items a wave claims per atomic
 constexpr uint32_t OUT_CLAIM = 64;  // survivor slots a wave reserves per atomic
 
@@ -809,6 +814,7 @@ struct psim_gpu_ctx {
     int64_t steps_cap = 0;
     int32_t window = 0;
     int32_t blocks_per_cu = 8;  // transport grid: n_CU x blocks_per_cu blocks
+    int32_t tally_mode = PSIM_TALLY_AUTO;  // where recorded steps are tallied
     // seed of the current run, read by the kernels from device memory (pinned host staging copy)
     uint64_t* d_seed = nullptr;
     uint64_t* h_seed = nullptr;
@@ -923,6 +929,17 @@ int32_t record_window_for(const psim_gpu_ctx* c) {
 
 int32_t free_window_for(const psim_gpu_ctx* c) { return c->window > 0 ? c->window : DEFAULT_FREE_WINDOW; }
 
+// Tally path of a run: LDS window partials for runs with enough phonons per sensor when enough
+// steps of tallies fit the LDS budget, else FP64 atomics into the HBM tallies (one launch for the
+// whole run).  psim_gpu_set_tally_mode forces either (LDS only where one step fits).
+bool use_global_tally(const psim_gpu_ctx* c) {
+    const int64_t sensors = std::max(1, c->n_sensors);
+    const int64_t lds_steps = LDS_TALLY_BYTES / (20 * sensors);
+    if (c->tally_mode == PSIM_TALLY_HBM || lds_steps < 1) { return true; }
+    if (c->tally_mode == PSIM_TALLY_LDS) { return false; }
+    return lds_steps < GLOBAL_TALLY_STEPS || c->params.total_phonons < LDS_MIN_PHONONS_PER_SENSOR * sensors;
+}
+
 // Launch windows [k0, k1) over N steps: step values k0+1..k1 are reached inside window [k0, k1);
 // windows whose step values all lie below adj (the first recorded step) are tally-free.
 std::vector<std::pair<int64_t, int64_t>> plan_windows(int64_t N, int64_t adj, int32_t w_free, int32_t w_rec) {
@@ -1035,7 +1052,7 @@ int prepare_run(psim_gpu_ctx* c, RunPlan& plan) {
         c->list_cap = (int64_t)cap;
         c->gen++;
     }
-    const bool global_tally = LDS_TALLY_BYTES / (20 * std::max(1, c->n_sensors)) < GLOBAL_TALLY_STEPS;
+    const bool global_tally = use_global_tally(c);
     const int32_t W = global_tally ? free_window_for(c) : record_window_for(c);
     const int64_t N = P.num_measurements;
     const auto windows = plan_windows(N, P.step_adjustment, free_window_for(c), W);
@@ -1555,6 +1572,13 @@ int psim_gpu_set_grid(psim_gpu_ctx* c, int32_t blocks_per_cu) {
     if (!c || blocks_per_cu < 1 || blocks_per_cu > 64) { return PSIM_E_INVALID; }
     if (c->blocks_per_cu != blocks_per_cu) { c->gen++; }
     c->blocks_per_cu = blocks_per_cu;
+    return PSIM_OK;
+}
+
+int psim_gpu_set_tally_mode(psim_gpu_ctx* c, int32_t mode) {
+    if (!c || mode < PSIM_TALLY_AUTO || mode > PSIM_TALLY_HBM) { return PSIM_E_INVALID; }
+    if (c->tally_mode != mode) { c->gen++; }
+    c->tally_mode = mode;
     return PSIM_OK;
 }
 

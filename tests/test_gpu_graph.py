@@ -22,6 +22,8 @@ CASES = {
                                                       step_interval=50).to_dict(),
     "lshape_steady": configs.lshape_full(num_phonons=20000, num_measurements=1000),
 }
+# the tally path each case is run on (the automatic rule would pick HBM atomics for all three)
+TALLY = {"bar_periodic_lds_tallies": "lds", "bar_400_sensors_hbm_tallies": "hbm", "lshape_steady": "lds"}
 
 
 @pytest.fixture(scope="module")
@@ -46,17 +48,23 @@ def _same(a, b):
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_graph_replays_match_eager(engine, name):
-    print(f"case {name}", flush=True)
+    print(f"case {name} tally {TALLY[name]}", flush=True)
     engine.set_window(0)
     engine.set_grid(8)
+    engine.set_tally_mode(TALLY[name])
     model = psim_amd.Model(CASES[name])
     model.upload(engine, 0)
-    for seed in (5, 6, 5, 7, 6):
-        g = engine.run_graph(model.num_sensors, model.num_record, seed)
-        _same(g, _eager(engine, model, seed))
-    engine.set_window(7)  # a new launch sequence: captured again
-    _same(engine.run_graph(model.num_sensors, model.num_record, 9), _eager(engine, model, 9))
-    engine.set_window(0)
+    try:
+        for seed in (5, 6, 5, 7, 6):
+            g = engine.run_graph(model.num_sensors, model.num_record, seed)
+            _same(g, _eager(engine, model, seed))
+        engine.set_window(7)  # a new launch sequence: captured again
+        _same(engine.run_graph(model.num_sensors, model.num_record, 9), _eager(engine, model, 9))
+        engine.set_tally_mode("hbm" if TALLY[name] == "lds" else "lds")  # the other path: captured again
+        _same(engine.run_graph(model.num_sensors, model.num_record, 9), _eager(engine, model, 9))
+    finally:
+        engine.set_window(0)
+        engine.set_tally_mode("auto")
 
 
 def test_autotune_keeps_results(engine):
@@ -64,6 +72,7 @@ def test_autotune_keeps_results(engine):
     model.upload(engine, 0)
     engine.set_window(0)
     engine.set_grid(8)
+    engine.set_tally_mode("lds")  # the window choice matters on the LDS path
     reference = _eager(engine, model, 3)
     choice = engine.autotune(seed=1)
     print("autotune choice", choice, flush=True)
@@ -72,6 +81,7 @@ def test_autotune_keeps_results(engine):
     _same(engine.run_graph(model.num_sensors, model.num_record, 3), reference)
     engine.set_window(0)
     engine.set_grid(8)
+    engine.set_tally_mode("auto")
 
 
 def _engine_hip_runtime():
@@ -93,6 +103,7 @@ def test_async_runs_on_a_caller_stream_match_eager(engine):
     hip = _engine_hip_runtime()
     engine.set_window(0)
     engine.set_grid(8)
+    engine.set_tally_mode("lds")
     model = psim_amd.Model(CASES["bar_periodic_lds_tallies"])
     model.upload(engine, 0)
     stream = ctypes.c_void_p()
@@ -107,3 +118,4 @@ def test_async_runs_on_a_caller_stream_match_eager(engine):
     engine.run_async(8, 0)  # the null stream
     assert hip.hipDeviceSynchronize() == 0
     _same(engine.fetch(model.num_sensors, model.num_record), _eager(engine, model, 8))
+    engine.set_tally_mode("auto")

@@ -57,14 +57,20 @@ def _flux_close(a, b):
     return np.max(np.abs(a - b)) <= 1e-9 * scale
 
 
+@pytest.mark.parametrize("tally", ["lds", "hbm"])
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_trajectories_match_oracle(engine, name):
+def test_trajectories_match_oracle(engine, name, tally):
+    """Both tally paths: LDS window partials and FP64 atomics into HBM."""
     cfg = CASES[name]
     seed = 11
-    print(f"case {name} seed {seed}", flush=True)
+    print(f"case {name} tally {tally} seed {seed}", flush=True)
     model = psim_amd.Model(cfg)
     engine.set_fate(True)
-    model.run(engine, seed)
+    engine.set_tally_mode(tally)
+    try:
+        model.run(engine, seed)
+    finally:
+        engine.set_tally_mode("auto")
     res = model.results(0)
     eff, total = model.budget()
     g_step, g_cause, g_xy = engine.fetch_fate(total)
@@ -93,6 +99,7 @@ def test_launch_window_does_not_change_results(engine, window):
     model = psim_amd.Model(cfg)
     engine.set_fate(True)
     engine.set_window(0)
+    engine.set_tally_mode("lds")  # windows of recorded steps in LDS
     model.run(engine, 5)
     base = model.results(0)
     _, total = model.budget()
