@@ -12,7 +12,10 @@
  *    argument order; here r1 is drawn first;
  *  - reads the reference would make out of bounds (a birth step >= the number
  *    of steps, a steady-state tally index >= the recorded length) are
- *    skipped and counted.
+ *    skipped and counted;
+ *  - in steady state a phonon crosses the measurement boundaries below
+ *    boundary step_adjustment - 1 without stopping (see simulate_phonon):
+ *    positions and times to scatter round differently, nothing else changes.
  */
 #define _POSIX_C_SOURCE 200809L
 #include "psim_oracle.h"
@@ -1171,22 +1174,54 @@ static int next_impact(const or_model* m, phonon_t* p, double time, double* out_
         p->px = impact.x;
         p->py = impact.y;
         ta->counters[OR_CNT_IMPACTS]++;
-        handle_surface_collision(m, p, impact, st);
         *out_time = time;
         return 1;
     }
     return 0;
 }
 
+/* A flight across untallied measurement boundaries (steady state, step < stop): the phonon's
+ * step counter, the boundary its flight ends at, the loop's event count and the flight's start
+ * time. */
+typedef struct {
+    int64_t* step;
+    int* events;
+    int64_t stop;
+    double age;
+} flight_t;
+
+/* Advances *step to the last boundary k < stop whose time k * sim_time / N (the reference's
+ * step_times_) is at or before t, never backwards; the stream moves to the new step.  Every
+ * boundary passed is a phonon-step.  Returns 1 when the step moved. */
+static int advance_step(const or_model* m, phonon_t* p, int64_t* step, int64_t stop, double t, psim_stream* st,
+    tally_t* ta) {
+    const int64_t N = m->num_measurements;
+    int64_t k = *step;
+    /* a start at or below the answer: floor(t / dt) - 1 lies a whole step below t */
+    const int64_t est = (int64_t)(t * ((double)N / m->sim_time)) - 1;
+    if (est > k) { k = (est < stop - 1) ? est : stop - 1; }
+    while (k + 1 < stop && (double)(k + 1) * m->sim_time / (double)N <= t) { ++k; }
+    if (k == *step) { return 0; }
+    ta->counters[OR_CNT_PHONON_STEPS] += k - *step;
+    *step = k;
+    p->lifestep = k;
+    psim_stream_set_step(st, (uint32_t)k);
+    return 1;
+}
+
 /* ModelSimulator::handleImpacts, modelSimulator.cpp:205-225.  Returns 0 when the phonon left
  * the system (std::nullopt), 1 otherwise with *drifted set. */
 static int handle_impacts(const or_model* m, phonon_t* p, double drift_time, int sensor, double* drifted_out,
-    psim_stream* st, tally_t* ta) {
+    psim_stream* st, tally_t* ta, const flight_t* fl) {
     double t;
     int hit = next_impact(m, p, drift_time, &t, st, ta);
     double drifted = 0.;
     int collisions = 0;
     while (hit) {
+        /* on a long flight the surface sees the step of the moment of impact */
+        if (fl && advance_step(m, p, fl->step, fl->stop, fl->age + (drifted + t), st, ta)) { *fl->events = 0; }
+        const pt poi = { p->px, p->py };
+        handle_surface_collision(m, p, poi, st);
         if (p->cell < 0) { return 0; }
         drifted += t;
         if (++collisions > MAX_COLLISIONS) {
@@ -1252,7 +1287,16 @@ static void tally(const or_model* m, const phonon_t* p, int64_t idx, tally_t* ta
     ta->counters[OR_CNT_TALLIES]++;
 }
 
-/* ModelSimulator::simulatePhonon, modelSimulator.cpp:139-198 */
+/* ModelSimulator::simulatePhonon, modelSimulator.cpp:139-198.
+ *
+ * In steady state the measurement boundaries below the first recorded step only advance the
+ * step counter (modelSimulator.cpp:182-188: nothing is recorded below step_adjustment_), so a
+ * phonon whose step lies below stop = step_adjustment - 1 flies across them: its flight ends at
+ * its scattering time, at an impact that changes sensor, or at boundary `stop`, and its step
+ * advances to the boundaries it passed (at each impact before the surface handles the phonon,
+ * and at the end of the flight), so the step of every draw and surface test is the step of the
+ * moment it happens.  From boundary `stop` on, every boundary stops the phonon as in the
+ * reference. */
 static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint32_t id, tally_t* ta,
     int64_t* fate_step, int32_t* fate_cause, double* fate_xy) {
     const int64_t N = m->num_measurements;
@@ -1270,6 +1314,8 @@ static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint3
         double time_to_scatter = 0.;
         double time_to_measurement = 0.;
         int events = 0;
+        const int64_t stop = m->step_adjustment >= 2 ? m->step_adjustment - 1 : 0;
+        const double stop_time = (double)stop * m->sim_time / (double)N;
         for (;;) {
             if (++events > MAX_EVENTS_PER_STEP) {
                 ta->counters[OR_CNT_STALLED]++;
@@ -1280,13 +1326,17 @@ static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint3
                 transport_rates(m, p, relax);
                 time_to_scatter = SCALING_FACTOR * -psim_log(psim_urand(&st)) / (0. + relax[0] + relax[1] + relax[2]);
             }
-            if (time_to_measurement <= 0.) {
+            const int merged = step < stop;
+            if (merged) {
+                time_to_measurement = stop_time - phonon_age;
+            } else if (time_to_measurement <= 0.) {
                 time_to_measurement = (double)(step + 1) * m->sim_time / (double)N - phonon_age;
             }
             double drift_time = (time_to_measurement < time_to_scatter) ? time_to_measurement : time_to_scatter;
             const int sensor = m->cells[p->cell].sensor;
             double drifted;
-            if (handle_impacts(m, p, drift_time, sensor, &drifted, &st, ta)) {
+            const flight_t fl = { &step, &events, stop, phonon_age };
+            if (handle_impacts(m, p, drift_time, sensor, &drifted, &st, ta, merged ? &fl : NULL)) {
                 if (m->cells[p->cell].sensor != sensor) { drift_time = drifted; }
                 const double factor = p->velocity * (drift_time - drifted); /* Phonon::drift */
                 p->px += p->dx * factor;
@@ -1294,7 +1344,13 @@ static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint3
                 phonon_age += drift_time;
                 time_to_measurement -= drift_time;
                 time_to_scatter -= drift_time;
-                if (time_to_measurement == 0.) {
+                if (time_to_measurement == 0. && merged) { /* boundary `stop`: nothing recorded below it */
+                    ta->counters[OR_CNT_PHONON_STEPS] += stop - step;
+                    events = 0;
+                    step = stop;
+                    p->lifestep = step;
+                    psim_stream_set_step(&st, (uint32_t)step);
+                } else if (time_to_measurement == 0.) {
                     ta->counters[OR_CNT_PHONON_STEPS]++;
                     events = 0;
                     if (++step < N) {
@@ -1306,11 +1362,14 @@ static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint3
                         cause = 2;
                         break;
                     }
-                } else if (!m->phasor && time_to_scatter == 0.) {
-                    ta->counters[OR_CNT_SCATTERS]++;
-                    scatter(m, p, relax, &st);
                 } else {
-                    time_to_scatter = 0.;
+                    if (merged && advance_step(m, p, &step, stop, phonon_age, &st, ta)) { events = 0; }
+                    if (!m->phasor && time_to_scatter == 0.) {
+                        ta->counters[OR_CNT_SCATTERS]++;
+                        scatter(m, p, relax, &st);
+                    } else {
+                        time_to_scatter = 0.;
+                    }
                 }
             } else {
                 ta->counters[OR_CNT_ABSORBED]++;

@@ -243,6 +243,7 @@ struct Live {
     uint32_t* __restrict__ meta;
     uint32_t* __restrict__ id;
     uint32_t* __restrict__ sid;  // id within the phonon's random stream
+    int32_t* __restrict__ step;  // measurement step (past the window's end after a long flight)
 };
 
 // Work of one transport launch: the started phonons (Live buffer `in`, *n_in entries) followed by
@@ -335,6 +336,10 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     int cur_sensor = 0;  // sensor of p.cell
     psim_stream st;
     psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, 0, 0);
+    // Long flights (steady state, step < stop): the flight ends at boundary `stop` at the latest.
+    const int32_t stop = m.stop;
+    const double stop_time = (double)stop * m.sim_time / (double)m.N;
+    const double steps_per_time = (double)m.N / m.sim_time;
     // Work-queue items the wave has claimed but not handed out yet (wave-uniform): a refill takes
     // them in order and claims QUEUE_CLAIM more with one atomic when they run out.
     uint32_t res_next = 0, res_end = 0;
@@ -344,7 +349,11 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     // stall bound, S_TTS when a scattering time is due (drawn once, then drift_setup), else
     // drift_setup directly.
     auto drift_setup = [&]() {
-        if (ttm <= 0.) { ttm = (double)(p.step + 1) * m.sim_time / (double)m.N - age; }
+        if (p.step < stop) {
+            ttm = stop_time - age;
+        } else if (ttm <= 0.) {
+            ttm = (double)(p.step + 1) * m.sim_time / (double)m.N - age;
+        }
         drift_time = (ttm < tts) ? ttm : tts;
         sensor0 = cur_sensor;
         drifted = 0.;
@@ -363,7 +372,26 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
         }
         drift_setup();
     };
+    // A long flight passed boundaries up to time t: the step moves to the last boundary k < stop
+    // with k * sim_time / N <= t (never backwards) and the stream to that step; every boundary
+    // passed is a phonon-step.  Returns whether the step moved.
+    auto advance_step = [&](double t) -> bool {
+        int32_t k = p.step;
+        const int32_t est = (int32_t)(t * steps_per_time) - 1;  // a whole step below t: at or below the answer
+        if (est > k) { k = est < stop - 1 ? est : stop - 1; }
+        while (k + 1 < stop && (double)(k + 1) * m.sim_time / (double)m.N <= t) { ++k; }
+        if (k == p.step) { return false; }
+        c.steps += (uint32_t)(k - p.step);
+        p.step = k;
+        events = 0;
+        psim_stream_set_step(&st, (uint32_t)k);
+        return true;
+    };
     // The rest of simulatePhonon's loop body once handleImpacts returned `result`.
+    // A long flight may carry the phonon past k1: it pauses right after a boundary crossing that
+    // left its stream at the start of a step with no draw made (so a resumed phonon continues
+    // exactly), and at the latest at boundary `stop`, where every window before the recorded
+    // steps ends.
     auto end_drift = [&](double result) {
         if (cur_sensor != sensor0) { drift_time = result; }
         const double factor = p.vel * (drift_time - result);  // Phonon::drift
@@ -372,7 +400,17 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
         age += drift_time;
         ttm -= drift_time;
         tts -= drift_time;
-        if (ttm == 0.) {
+        const bool long_flight = p.step < stop;
+        if (ttm == 0. && long_flight) {  // boundary `stop`; nothing is recorded below adj
+            c.steps += (uint32_t)(stop - p.step);
+            p.step = stop;
+            events = 0;
+            psim_stream_set_step(&st, (uint32_t)stop);
+            if (p.step >= k1) {
+                state = S_PAUSED;
+                return;
+            }
+        } else if (ttm == 0.) {
             c.steps++;
             events = 0;
             if (++p.step < m.N) {
@@ -405,11 +443,17 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                 state = S_DEAD + FATE_EXPIRED * 8;
                 return;
             }
-        } else if (!m.phasor && tts == 0.) {
-            state = S_SCAT;
-            return;
         } else {
+            const bool advanced = long_flight && advance_step(age);
+            if (!m.phasor && tts == 0.) {
+                state = S_SCAT;
+                return;
+            }
             tts = 0.;
+            if (advanced && p.step >= k1) {
+                state = S_PAUSED;
+                return;
+            }
         }
         begin_event();
     };
@@ -480,14 +524,18 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                     p.bin = (int)(meta & META_BIN_MASK);
                     p.pol = (meta & META_TA) ? POL_TA : POL_LA;
                     p.sign = (meta & META_NEG) ? -1 : 1;
-                    p.step = started ? k0 : (int32_t)(age / m.step_time);
+                    p.step = started ? L.step[item] : (int32_t)(age / m.step_time);
                     ttm = 0.;
                     events = 0;
                     // the stream keeps (stream index, id within it) in its counter words c3 and c2
                     psim_stream_init(&st, seed,
                         MULTI ? psim_stream_tag(PSIM_TAG_TRANSPORT, meta >> META_STREAM_SHIFT) : PSIM_TAG_TRANSPORT, sid,
                         (uint32_t)p.step);
-                    begin_event();
+                    if (p.step >= k1) {  // a long flight already carried it past this window
+                        state = S_PAUSED;
+                    } else {
+                        begin_event();
+                    }
                 }
             }
         } else if (sel == S_NI) {
@@ -500,6 +548,8 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
             }
         } else if (sel == S_COLL) {
             if (state == S_COLL) {  // surface handling, then handleImpacts' bookkeeping
+                // on a long flight the surface sees the step of the moment of impact
+                if (p.step < stop) { advance_step(age + (drifted + t_hit)); }
                 handle_surface_collision(m, p, p.px, p.py, st);
                 if (p.cell < 0) {
                     c.absorbed++;
@@ -583,6 +633,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                 O.meta[k] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
                             | META_STARTED | (MULTI ? (st.c3 >> 8) << META_STREAM_SHIFT : 0u);
                 O.id[k] = slot;
+                O.step[k] = p.step;
                 if (MULTI) { O.sid[k] = st.c2; }
                 state = S_FETCH;
             }
@@ -718,6 +769,7 @@ __global__ void __launch_bounds__(SORT_BLOCK) cell_scatter_kernel(Live src, Live
         dst.sensor[k] = src.sensor[i];
         dst.meta[k] = src.meta[i];
         dst.id[k] = src.id[i];
+        dst.step[k] = src.step[i];
         if (copy_sid) { dst.sid[k] = src.sid[i]; }
     }
 }
@@ -873,7 +925,7 @@ void free_lists(psim_gpu_ctx* c) {
     }
     for (Live& L : c->live) {
         void* ls[] = { L.px, L.py, L.dx, L.dy, L.freq, L.vel, L.age, L.tts, L.rate0, L.rate1, L.rate2, L.cell, L.sensor,
-            L.meta, L.id, L.sid };
+            L.meta, L.id, L.sid, L.step };
         for (void* p : ls) {
             if (p) { (void)hipFree(p); }
         }
@@ -1042,8 +1094,9 @@ int prepare_run(psim_gpu_ctx* c, RunPlan& plan) {
             L.meta = dalloc<uint32_t>(cap);
             L.id = dalloc<uint32_t>(cap);
             L.sid = dalloc<uint32_t>(cap);
+            L.step = dalloc<int32_t>(cap);
             ok = ok && L.px && L.py && L.dx && L.dy && L.freq && L.vel && L.age && L.tts && L.rate0 && L.rate1 && L.rate2
-                 && L.cell && L.sensor && L.meta && L.id && L.sid;
+                 && L.cell && L.sensor && L.meta && L.id && L.sid && L.step;
         }
         if (!ok) {
             free_lists(c);
@@ -1477,6 +1530,7 @@ int psim_gpu_upload(psim_gpu_ctx* c, const psim_geom* g, const psim_tables* t, c
     m.phasor = p->phasor;
     m.N = (int32_t)p->num_measurements;
     m.adj = (int32_t)p->step_adjustment;
+    m.stop = p->step_adjustment >= 2 ? (int32_t)(p->step_adjustment - 1) : 0;
     m.R = (int32_t)p->num_record;
     m.sim_time = p->sim_time;
     m.step_time = p->step_time;

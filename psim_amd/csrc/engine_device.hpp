@@ -63,6 +63,9 @@ struct Model {
     int32_t N;  // measurement steps (validated to fit int32 at upload)
     int32_t adj;
     int32_t R;
+    // Steady state: phonons whose step lies below `stop` (= adj - 1, else 0) fly across the
+    // measurement boundaries below it without stopping; those boundaries only count steps.
+    int32_t stop;
     double sim_time;
     double step_time;
     double t_eq;
