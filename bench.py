@@ -205,7 +205,7 @@ def batch_bench(args) -> dict:
     for w in range(max(1, args.warmup)):
         psim_amd.run_batch(engine, models, seed=10_000 + w)
     # launch shape for this batch (the batch is now uploaded); no choice changes any result
-    tune = engine.autotune(seed=20_000)
+    tune = None if args.no_autotune else engine.autotune(seed=20_000)
     gpu_s = wall = 0.0
     steps_total = 0
     for k in range(args.steps):
@@ -228,7 +228,7 @@ def batch_bench(args) -> dict:
     engine.set_window(0)
     engine.set_grid(8)
     seq_steps, seq_gpu = 0, 0.0
-    for m in models:
+    for m in ([] if args.no_sequential else models):
         m.run(engine, 0)
         seq_steps += m.results(0).counters["phonon_steps"]
         seq_gpu += engine.last_timing()["run_ms"] / 1e3
@@ -244,8 +244,9 @@ def batch_bench(args) -> dict:
         "batch": {"gpu_run_s": gpu_s / args.steps, "end_to_end_s": wall / args.steps,
                   "host_model_build_s": t_build, "phonon_steps": steps_total // args.steps,
                   "autotune": tune, "graph_replay": {"value": g_steps / g_s, "gpu_run_s": g_s / args.steps}},
-        "sequential_gpu": {"value": seq_steps / seq_gpu, "gpu_run_s": seq_gpu, "phonon_steps": seq_steps,
-                           "batch_speedup": value / (seq_steps / seq_gpu)},
+        "sequential_gpu": None if args.no_sequential else {
+            "value": seq_steps / seq_gpu, "gpu_run_s": seq_gpu, "phonon_steps": seq_steps,
+            "batch_speedup": value / (seq_steps / seq_gpu)},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
@@ -354,6 +355,10 @@ def main() -> int:
     ap.add_argument("--cpu-sample-phonons", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-steady-state", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batch workload: keep the default launch shape (for kernel traces of batched runs only)")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batch workload: skip the one-by-one runs of the same models")
     args = ap.parse_args()
     film = args.workload == "film"
     if args.steps is None:

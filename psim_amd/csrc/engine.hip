@@ -337,7 +337,9 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     psim_stream st;
     psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, 0, 0);
     // Long flights (steady state, step < stop): the flight ends at boundary `stop` at the latest.
-    const int32_t stop = m.stop;
+    // Only untallied launches hold such phonons: every window that records starts at or after
+    // step adj - 1 = stop, so the tallied variants carry no long-flight code.
+    const int32_t stop = TALLY == TALLY_NONE ? m.stop : 0;
     const double stop_time = (double)stop * m.sim_time / (double)m.N;
     const double steps_per_time = (double)m.N / m.sim_time;
     // Work-queue items the wave has claimed but not handed out yet (wave-uniform): a refill takes
