@@ -48,6 +48,7 @@ def _load():
     lib.or_add_cell.argtypes = [vp, _D, ctypes.c_int64, ctypes.c_double]
     lib.or_add_emit.argtypes = [vp, _D]
     lib.or_prepare.argtypes = [vp, ctypes.c_uint64]
+    lib.or_set_step_by_step.argtypes = [vp, ctypes.c_int]
     lib.or_num_sensors.argtypes = [vp]
     lib.or_num_record.restype = ctypes.c_int64
     lib.or_num_record.argtypes = [vp]
@@ -173,9 +174,12 @@ class Model:
         self._lib.or_sources(self._h, _p(cell, _I64), _p(kind, _I32), _p(energy, _D), _p(count, _I64))
         return {"cell": cell, "kind": kind, "energy": energy, "count": count}
 
-    def run(self, seed: int, nthreads: int = 1, fate: bool = False) -> RunResult:
+    def run(self, seed: int, nthreads: int = 1, fate: bool = False, step_by_step: bool = False) -> RunResult:
+        """step_by_step: the reference's loop as written (one drift per measurement step) in place
+        of flights across the boundaries; the two agree up to rounding."""
         if self.seed != int(seed):
             self.prepare(seed)
+        self._lib.or_set_step_by_step(self._h, int(bool(step_by_step)))
         S, R = self.num_sensors, self.num_record
         counts = np.zeros((S, R), np.int64)
         flux = np.zeros((S, R, 2), np.float64)

@@ -13,8 +13,8 @@
  *  - reads the reference would make out of bounds (a birth step >= the number
  *    of steps, a steady-state tally index >= the recorded length) are
  *    skipped and counted;
- *  - in steady state a phonon crosses the measurement boundaries below
- *    boundary step_adjustment - 1 without stopping (see simulate_phonon):
+ *  - a phonon does not stop at every measurement boundary: it is counted and
+ *    tallied at each boundary its flight passes (see simulate_phonon), so
  *    positions and times to scatter round differently, nothing else changes.
  */
 #define _POSIX_C_SOURCE 200809L
@@ -40,6 +40,10 @@
 #define SCALING_FACTOR 1e9
 #define VELOCITY_EPS 0.01
 #define MAX_COLLISIONS 100
+/* Flights through the recorded steps also end at every FLIGHT_STEPS-th boundary (counted from
+ * the first boundary of the recorded stretch), so that a window of a multiple of it holds whole
+ * flights; the engine uses the same constant. */
+#define FLIGHT_STEPS 8
 #define TEMP_BOUND_EPS 10.0
 #define PHASOR_TEMP_BOUND_EPS (TEMP_BOUND_EPS * 100.0)
 #define SS_STEPS_PERCENT 0.1
@@ -415,6 +419,7 @@ struct or_model {
     int sim_type, phasor;
     double step_time;
     int64_t step_adjustment; /* ModelSimulator::step_adjustment_ */
+    int step_by_step; /* the reference's loop as written: one drift per measurement step */
     int64_t start_step; /* Model::start_step_ */
     int64_t n_record; /* Sensor tally length */
     int n_mat, cap_mat;
@@ -903,6 +908,11 @@ static double cell_emit_energy(or_model* m, int ci, double t_eq) {
 
 /* Model::runSimulation preamble (model.cpp:141-153) and ModelSimulator::initPhononBuilders
  * (modelSimulator.cpp:43-85), with the Bernoulli draw of source s taken from the budget stream. */
+int or_set_step_by_step(or_model* m, int on) {
+    m->step_by_step = on != 0;
+    return 0;
+}
+
 int or_prepare(or_model* m, uint64_t seed) {
     if (m->n_cells == 0 || m->n_sensors == 0) { return set_err(m, "model has no cells or sensors"); }
     /* Model::setTemperatureBounds, model.cpp:203-218 */
@@ -1180,28 +1190,36 @@ static int next_impact(const or_model* m, phonon_t* p, double time, double* out_
     return 0;
 }
 
-/* A flight across untallied measurement boundaries (steady state, step < stop): the phonon's
- * step counter, the boundary its flight ends at, the loop's event count and the flight's start
- * time. */
+/* A flight: the phonon's step counter, the boundary its flight ends at (`cap`), the loop's
+ * event count and the flight's start time. */
 typedef struct {
     int64_t* step;
     int* events;
-    int64_t stop;
+    int64_t cap;
     double age;
+    int step_by_step;
 } flight_t;
 
-/* Advances *step to the last boundary k < stop whose time k * sim_time / N (the reference's
- * step_times_) is at or before t, never backwards; the stream moves to the new step.  Every
- * boundary passed is a phonon-step.  Returns 1 when the step moved. */
-static int advance_step(const or_model* m, phonon_t* p, int64_t* step, int64_t stop, double t, psim_stream* st,
+static void tally(const or_model* m, const phonon_t* p, int64_t idx, tally_t* ta);
+
+/* The measurement events of the boundaries a flight passed up to time t
+ * (modelSimulator.cpp:182-188): *step advances to the last boundary k < cap whose time
+ * k * sim_time / N (the reference's step_times_) is at or before t, never backwards, and the
+ * stream moves to the new step.  Every boundary passed is a phonon-step, and from
+ * step_adjustment on it is tallied with the phonon's sensor and velocity of that moment
+ * (Sensor::updateHeatParams, sensor.cpp:43-52).  Returns 1 when the step moved. */
+static int advance_step(const or_model* m, phonon_t* p, int64_t* step, int64_t cap, double t, psim_stream* st,
     tally_t* ta) {
     const int64_t N = m->num_measurements;
     int64_t k = *step;
     /* a start at or below the answer: floor(t / dt) - 1 lies a whole step below t */
     const int64_t est = (int64_t)(t * ((double)N / m->sim_time)) - 1;
-    if (est > k) { k = (est < stop - 1) ? est : stop - 1; }
-    while (k + 1 < stop && (double)(k + 1) * m->sim_time / (double)N <= t) { ++k; }
+    if (est > k) { k = (est < cap - 1) ? est : cap - 1; }
+    while (k + 1 < cap && (double)(k + 1) * m->sim_time / (double)N <= t) { ++k; }
     if (k == *step) { return 0; }
+    for (int64_t j = (*step + 1 > m->step_adjustment ? *step + 1 : m->step_adjustment); j <= k; ++j) {
+        tally(m, p, j - m->step_adjustment, ta);
+    }
     ta->counters[OR_CNT_PHONON_STEPS] += k - *step;
     *step = k;
     p->lifestep = k;
@@ -1212,14 +1230,18 @@ static int advance_step(const or_model* m, phonon_t* p, int64_t* step, int64_t s
 /* ModelSimulator::handleImpacts, modelSimulator.cpp:205-225.  Returns 0 when the phonon left
  * the system (std::nullopt), 1 otherwise with *drifted set. */
 static int handle_impacts(const or_model* m, phonon_t* p, double drift_time, int sensor, double* drifted_out,
-    psim_stream* st, tally_t* ta, const flight_t* fl) {
+    int* relocated, psim_stream* st, tally_t* ta, const flight_t* fl) {
     double t;
     int hit = next_impact(m, p, drift_time, &t, st, ta);
     double drifted = 0.;
     int collisions = 0;
     while (hit) {
-        /* on a long flight the surface sees the step of the moment of impact */
-        if (fl && advance_step(m, p, fl->step, fl->stop, fl->age + (drifted + t), st, ta)) { *fl->events = 0; }
+        /* the boundaries passed before the impact see the old direction; the surface sees the
+         * step of the moment of impact */
+        if (advance_step(m, p, fl->step, fl->cap, fl->age + (drifted + t), st, ta)) {
+            *fl->events = 0;
+            collisions = 0; /* the reference counts collisions per drift, and a drift ends at a boundary */
+        }
         const pt poi = { p->px, p->py };
         handle_surface_collision(m, p, poi, st);
         if (p->cell < 0) { return 0; }
@@ -1231,7 +1253,11 @@ static int handle_impacts(const or_model* m, phonon_t* p, double drift_time, int
             p->px = q.x;
             p->py = q.y;
             ta->counters[OR_CNT_RELOCATIONS]++;
-            *drifted_out = drift_time;
+            /* the phonon stays at the new point for the rest of the reference's drift: up to the
+             * next boundary, or the flight's end if that comes first */
+            const double frozen = (double)(*fl->step + 1) * m->sim_time / (double)m->num_measurements - fl->age;
+            *drifted_out = (!fl->step_by_step && frozen < drift_time) ? frozen : drift_time;
+            *relocated = 1;
             return 1;
         }
         if (m->cells[p->cell].sensor != sensor) {
@@ -1287,16 +1313,19 @@ static void tally(const or_model* m, const phonon_t* p, int64_t idx, tally_t* ta
     ta->counters[OR_CNT_TALLIES]++;
 }
 
-/* ModelSimulator::simulatePhonon, modelSimulator.cpp:139-198.
+/* ModelSimulator::simulatePhonon, modelSimulator.cpp:139-198, with flights in place of one
+ * drift per measurement step.
  *
- * In steady state the measurement boundaries below the first recorded step only advance the
- * step counter (modelSimulator.cpp:182-188: nothing is recorded below step_adjustment_), so a
- * phonon whose step lies below stop = step_adjustment - 1 flies across them: its flight ends at
- * its scattering time, at an impact that changes sensor, or at boundary `stop`, and its step
- * advances to the boundaries it passed (at each impact before the surface handles the phonon,
- * and at the end of the flight), so the step of every draw and surface test is the step of the
- * moment it happens.  From boundary `stop` on, every boundary stops the phonon as in the
- * reference. */
+ * A measurement event changes nothing about the phonon's motion (modelSimulator.cpp:182-188:
+ * the step advances and, from step_adjustment_ on, the sensor records the phonon's sign and
+ * velocity), so the phonon does not stop at each boundary.  Its flight ends at its scattering
+ * time, at an impact that changes sensor, or at its cap: boundary step_adjustment - 1 in steady
+ * state while the step lies below it (the recorded steps start there), else boundary N, where
+ * it dies; from `stop` on it also ends at every FLIGHT_STEPS-th boundary after `stop`.  The
+ * boundaries a flight passes are handled as events in time order: at each impact
+ * before the surface acts (so they see the direction before it and the surface sees the step
+ * of the impact) and at the end of the flight (advance_step).  Every draw therefore comes from
+ * the stream of the step of its moment. */
 static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint32_t id, tally_t* ta,
     int64_t* fate_step, int32_t* fate_cause, double* fate_xy) {
     const int64_t N = m->num_measurements;
@@ -1315,7 +1344,7 @@ static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint3
         double time_to_measurement = 0.;
         int events = 0;
         const int64_t stop = m->step_adjustment >= 2 ? m->step_adjustment - 1 : 0;
-        const double stop_time = (double)stop * m->sim_time / (double)N;
+
         for (;;) {
             if (++events > MAX_EVENTS_PER_STEP) {
                 ta->counters[OR_CNT_STALLED]++;
@@ -1326,48 +1355,52 @@ static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint3
                 transport_rates(m, p, relax);
                 time_to_scatter = SCALING_FACTOR * -psim_log(psim_urand(&st)) / (0. + relax[0] + relax[1] + relax[2]);
             }
-            const int merged = step < stop;
-            if (merged) {
-                time_to_measurement = stop_time - phonon_age;
-            } else if (time_to_measurement <= 0.) {
-                time_to_measurement = (double)(step + 1) * m->sim_time / (double)N - phonon_age;
+            int64_t cap;
+            if (m->step_by_step) { /* modelSimulator.cpp:159: the next boundary, kept until it is reached */
+                cap = step + 1;
+                if (time_to_measurement <= 0.) {
+                    time_to_measurement = (double)(step + 1) * m->sim_time / (double)N - phonon_age;
+                }
+            } else {
+                if (step < stop) {
+                    cap = stop;
+                } else {
+                    cap = stop + FLIGHT_STEPS * ((step - stop) / FLIGHT_STEPS + 1);
+                    if (cap > N) { cap = N; }
+                }
+                time_to_measurement = (double)cap * m->sim_time / (double)N - phonon_age;
             }
             double drift_time = (time_to_measurement < time_to_scatter) ? time_to_measurement : time_to_scatter;
             const int sensor = m->cells[p->cell].sensor;
             double drifted;
-            const flight_t fl = { &step, &events, stop, phonon_age };
-            if (handle_impacts(m, p, drift_time, sensor, &drifted, &st, ta, merged ? &fl : NULL)) {
-                if (m->cells[p->cell].sensor != sensor) { drift_time = drifted; }
+            int relocated = 0;
+            const flight_t fl = { &step, &events, cap, phonon_age, m->step_by_step };
+            if (handle_impacts(m, p, drift_time, sensor, &drifted, &relocated, &st, ta, &fl)) {
+                if (m->cells[p->cell].sensor != sensor || relocated) { drift_time = drifted; }
                 const double factor = p->velocity * (drift_time - drifted); /* Phonon::drift */
                 p->px += p->dx * factor;
                 p->py += p->dy * factor;
                 phonon_age += drift_time;
                 time_to_measurement -= drift_time;
                 time_to_scatter -= drift_time;
-                if (time_to_measurement == 0. && merged) { /* boundary `stop`: nothing recorded below it */
-                    ta->counters[OR_CNT_PHONON_STEPS] += stop - step;
-                    events = 0;
-                    step = stop;
-                    p->lifestep = step;
-                    psim_stream_set_step(&st, (uint32_t)step);
-                } else if (time_to_measurement == 0.) {
+                if (advance_step(m, p, &step, cap, phonon_age, &st, ta)) { events = 0; }
+                if (time_to_measurement == 0. && cap == N) { /* boundary N: the run is over */
+                    ta->counters[OR_CNT_PHONON_STEPS]++;
+                    ta->counters[OR_CNT_EXPIRED]++;
+                    cause = 2;
+                    break;
+                } else if (time_to_measurement == 0.) { /* boundary `cap` */
                     ta->counters[OR_CNT_PHONON_STEPS]++;
                     events = 0;
-                    if (++step < N) {
-                        p->lifestep = step;
-                        psim_stream_set_step(&st, (uint32_t)step);
-                        if (step >= m->step_adjustment) { tally(m, p, step - m->step_adjustment, ta); }
-                    } else {
-                        ta->counters[OR_CNT_EXPIRED]++;
-                        cause = 2;
-                        break;
-                    }
+                    step = cap;
+                    p->lifestep = step;
+                    psim_stream_set_step(&st, (uint32_t)step);
+                    if (step >= m->step_adjustment) { tally(m, p, step - m->step_adjustment, ta); }
                 } else {
-                    if (merged && advance_step(m, p, &step, stop, phonon_age, &st, ta)) { events = 0; }
                     if (!m->phasor && time_to_scatter == 0.) {
                         ta->counters[OR_CNT_SCATTERS]++;
                         scatter(m, p, relax, &st);
-                    } else {
+                    } else if (!relocated) { /* a new sensor: a new scattering time */
                         time_to_scatter = 0.;
                     }
                 }

@@ -44,6 +44,9 @@ int or_add_emit(or_model* m, const double* e);
 
 /* Bounds, material tables, effective energy and the seeded emission budget. */
 int or_prepare(or_model* m, uint64_t seed);
+/* on != 0: run the reference's transport loop as written, one drift per measurement step,
+ * instead of flights across the boundaries (a check of the flights; never used otherwise). */
+int or_set_step_by_step(or_model* m, int on);
 
 int or_num_sensors(const or_model* m);
 int64_t or_num_record(const or_model* m);

@@ -336,12 +336,17 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     int cur_sensor = 0;  // sensor of p.cell
     psim_stream st;
     psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, 0, 0);
-    // Long flights (steady state, step < stop): the flight ends at boundary `stop` at the latest.
-    // Only untallied launches hold such phonons: every window that records starts at or after
-    // step adj - 1 = stop, so the tallied variants carry no long-flight code.
+    // A flight ends at boundary `stop` (steady state, while the step lies below it), else at the
+    // next multiple of FLIGHT_STEPS boundaries after m.stop, or at boundary N.  Only untallied
+    // launches hold phonons below `stop`: every window that records starts at step adj - 1 = stop
+    // or later.
     const int32_t stop = TALLY == TALLY_NONE ? m.stop : 0;
-    const double stop_time = (double)stop * m.sim_time / (double)m.N;
     const double steps_per_time = (double)m.N / m.sim_time;
+    auto cap_of = [&](int32_t step) -> int32_t {
+        if (step < stop) { return stop; }
+        const int32_t cap = m.stop + FLIGHT_STEPS * ((step - m.stop) / FLIGHT_STEPS + 1);
+        return cap < m.N ? cap : m.N;
+    };
     // Work-queue items the wave has claimed but not handed out yet (wave-uniform): a refill takes
     // them in order and claims QUEUE_CLAIM more with one atomic when they run out.
     uint32_t res_next = 0, res_end = 0;
@@ -351,11 +356,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     // stall bound, S_TTS when a scattering time is due (drawn once, then drift_setup), else
     // drift_setup directly.
     auto drift_setup = [&]() {
-        if (p.step < stop) {
-            ttm = stop_time - age;
-        } else if (ttm <= 0.) {
-            ttm = (double)(p.step + 1) * m.sim_time / (double)m.N - age;
-        }
+        ttm = (double)cap_of(p.step) * m.sim_time / (double)m.N - age;  // time to the flight's cap
         drift_time = (ttm < tts) ? ttm : tts;
         sensor0 = cur_sensor;
         drifted = 0.;
@@ -374,26 +375,55 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
         }
         drift_setup();
     };
-    // A long flight passed boundaries up to time t: the step moves to the last boundary k < stop
-    // with k * sim_time / N <= t (never backwards) and the stream to that step; every boundary
-    // passed is a phonon-step.  Returns whether the step moved.
-    auto advance_step = [&](double t) -> bool {
+    // Sensor::updateHeatParams (sensor.cpp:43-52) for boundary k: the window's LDS tallies for
+    // k in (k0, k1], the HBM tallies for a boundary a flight passed beyond the window.
+    auto tally_boundary = [&](int32_t k) {
+        if constexpr (TALLY != TALLY_NONE) {
+            const int idx = k - m.adj;
+            if (idx < m.R && tally.rec) {
+                const double sg = (double)p.sign;
+                if (TALLY == TALLY_GLOBAL || k > k1) {
+                    const int64_t t = (int64_t)cur_sensor * m.R + idx;
+                    atomicAdd(reinterpret_cast<unsigned long long*>(&tally.counts[t]),
+                        (unsigned long long)(int64_t)p.sign);
+                    atomicAdd(&tally.flux[2 * t], p.dx * p.vel * sg);
+                    atomicAdd(&tally.flux[2 * t + 1], p.dy * p.vel * sg);
+                } else {
+                    const int e = (k - k0 - 1) * S + cur_sensor;
+                    atomicAdd(&lcnt[e], p.sign);
+                    atomicAdd(&lfx[e], p.dx * p.vel * sg);
+                    atomicAdd(&lfy[e], p.dy * p.vel * sg);
+                }
+                c.tallies++;
+            }
+        }
+    };
+    // The measurement events of the boundaries a flight passed up to time t
+    // (modelSimulator.cpp:182-188): the step moves to the last boundary k < cap with
+    // k * sim_time / N <= t (never backwards) and the stream to that step; every boundary passed
+    // is a phonon-step and, from adj on, a tally with the sensor and velocity of that moment.
+    // Returns whether the step moved.
+    auto advance_step = [&](double t, int32_t cap) -> bool {
         int32_t k = p.step;
         const int32_t est = (int32_t)(t * steps_per_time) - 1;  // a whole step below t: at or below the answer
-        if (est > k) { k = est < stop - 1 ? est : stop - 1; }
-        while (k + 1 < stop && (double)(k + 1) * m.sim_time / (double)m.N <= t) { ++k; }
+        if (est > k) { k = est < cap - 1 ? est : cap - 1; }
+        while (k + 1 < cap && (double)(k + 1) * m.sim_time / (double)m.N <= t) { ++k; }
         if (k == p.step) { return false; }
+        if constexpr (TALLY != TALLY_NONE) {
+            for (int32_t j = (p.step + 1 > m.adj ? p.step + 1 : m.adj); j <= k; ++j) { tally_boundary(j); }
+        }
         c.steps += (uint32_t)(k - p.step);
         p.step = k;
         events = 0;
         psim_stream_set_step(&st, (uint32_t)k);
         return true;
     };
-    // The rest of simulatePhonon's loop body once handleImpacts returned `result`.
-    // A long flight may carry the phonon past k1: it pauses right after a boundary crossing that
-    // left its stream at the start of a step with no draw made (so a resumed phonon continues
-    // exactly), and at the latest at boundary `stop`, where every window before the recorded
-    // steps ends.
+    // The rest of simulatePhonon's loop body once handleImpacts returned `result`, with the
+    // measurement events of the boundaries the flight passed.  A flight may carry the phonon past
+    // k1: it pauses right after a boundary crossing that left its stream at the start of a step
+    // with no draw made (so a resumed phonon continues exactly), at the latest at boundary
+    // `stop`, where every window before the recorded steps ends.
+    bool relocated = false;  // the flight ended frozen after a relocation (same sensor, same scattering time)
     auto end_drift = [&](double result) {
         if (cur_sensor != sensor0) { drift_time = result; }
         const double factor = p.vel * (drift_time - result);  // Phonon::drift
@@ -402,56 +432,33 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
         age += drift_time;
         ttm -= drift_time;
         tts -= drift_time;
-        const bool long_flight = p.step < stop;
-        if (ttm == 0. && long_flight) {  // boundary `stop`; nothing is recorded below adj
-            c.steps += (uint32_t)(stop - p.step);
-            p.step = stop;
+        const int32_t cap = cap_of(p.step);
+        const bool advanced = advance_step(age, cap);
+        if (ttm == 0. && cap == m.N) {  // boundary N: the run is over (step_times_ has N entries)
+            c.steps++;
+            c.expired++;
+            p.step = m.N;
+            state = S_DEAD + FATE_EXPIRED * 8;
+            return;
+        }
+        if (ttm == 0.) {  // the flight reached boundary `cap`
+            c.steps++;
+            if constexpr (TALLY != TALLY_NONE) {
+                if (cap >= m.adj) { tally_boundary(cap); }
+            }
+            p.step = cap;
             events = 0;
-            psim_stream_set_step(&st, (uint32_t)stop);
+            psim_stream_set_step(&st, (uint32_t)cap);
             if (p.step >= k1) {
                 state = S_PAUSED;
                 return;
             }
-        } else if (ttm == 0.) {
-            c.steps++;
-            events = 0;
-            if (++p.step < m.N) {
-                psim_stream_set_step(&st, (uint32_t)p.step);
-                if (TALLY != TALLY_NONE && p.step >= m.adj) {  // Sensor::updateHeatParams, sensor.cpp:43-52
-                    const int idx = p.step - m.adj;
-                    if (idx < m.R && tally.rec) {
-                        const double sg = (double)p.sign;
-                        if constexpr (TALLY == TALLY_GLOBAL) {
-                            const int64_t t = (int64_t)cur_sensor * m.R + idx;
-                            atomicAdd(reinterpret_cast<unsigned long long*>(&tally.counts[t]),
-                                (unsigned long long)(int64_t)p.sign);
-                            atomicAdd(&tally.flux[2 * t], p.dx * p.vel * sg);
-                            atomicAdd(&tally.flux[2 * t + 1], p.dy * p.vel * sg);
-                        } else {
-                            const int e = (p.step - k0 - 1) * S + cur_sensor;
-                            atomicAdd(&lcnt[e], p.sign);
-                            atomicAdd(&lfx[e], p.dx * p.vel * sg);
-                            atomicAdd(&lfy[e], p.dy * p.vel * sg);
-                        }
-                        c.tallies++;
-                    }
-                }
-                if (p.step == k1) {
-                    state = S_PAUSED;
-                    return;
-                }
-            } else {
-                c.expired++;
-                state = S_DEAD + FATE_EXPIRED * 8;
-                return;
-            }
         } else {
-            const bool advanced = long_flight && advance_step(age);
             if (!m.phasor && tts == 0.) {
                 state = S_SCAT;
                 return;
             }
-            tts = 0.;
+            if (!relocated) { tts = 0.; }  // a new sensor: a new scattering time
             if (advanced && p.step >= k1) {
                 state = S_PAUSED;
                 return;
@@ -550,8 +557,10 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
             }
         } else if (sel == S_COLL) {
             if (state == S_COLL) {  // surface handling, then handleImpacts' bookkeeping
-                // on a long flight the surface sees the step of the moment of impact
-                if (p.step < stop) { advance_step(age + (drifted + t_hit)); }
+                // the boundaries passed before the impact see the old direction; the surface sees
+                // the step of the moment of impact
+                // the reference counts collisions per drift, and a drift ends at a boundary
+                if (advance_step(age + (drifted + t_hit), cap_of(p.step))) { collisions = 0; }
                 handle_surface_collision(m, p, p.px, p.py, st);
                 if (p.cell < 0) {
                     c.absorbed++;
@@ -570,7 +579,13 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                         p.px = cl.x1 + (cl.x2 - cl.x1) * r1 + (cl.x3 - cl.x1) * r2;
                         p.py = cl.y1 + (cl.y2 - cl.y1) * r1 + (cl.y3 - cl.y1) * r2;
                         c.relocations++;
+                        // the phonon stays at the new point for the rest of the reference's drift:
+                        // up to the next boundary, or the flight's end if that comes first
+                        const double frozen = (double)(p.step + 1) * m.sim_time / (double)m.N - age;
+                        if (frozen < drift_time) { drift_time = frozen; }
+                        relocated = true;
                         end_drift(drift_time);
+                        relocated = false;
                     } else if (cur_sensor != sensor0) {
                         end_drift(drifted);
                     } else {
@@ -978,7 +993,9 @@ constexpr int32_t DEFAULT_FREE_WINDOW = 1 << 30;
 int32_t record_window_for(const psim_gpu_ctx* c) {
     const int64_t by_lds = std::max<int64_t>(1, LDS_TALLY_BYTES / (20 * std::max(1, c->n_sensors)));
     if (c->window > 0) { return (int32_t)std::min<int64_t>(c->window, std::max<int64_t>(by_lds, 1)); }
-    return (int32_t)std::min<int64_t>(64, by_lds);
+    // A multiple of FLIGHT_STEPS holds whole flights: no tally spills past the window into HBM.
+    const int64_t w = std::min<int64_t>(64, by_lds);
+    return (int32_t)(w >= FLIGHT_STEPS ? w - w % FLIGHT_STEPS : w);
 }
 
 int32_t free_window_for(const psim_gpu_ctx* c) { return c->window > 0 ? c->window : DEFAULT_FREE_WINDOW; }

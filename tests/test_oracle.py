@@ -299,6 +299,28 @@ def test_oracle_statistics_match_reference(name):
         assert np.all(z <= 3.0), (key, np.round(z, 2))
 
 
+@pytest.mark.parametrize("name", ["lshape_r", "film_r", "needles", "linear_ss"])
+def test_flights_agree_with_the_step_by_step_loop(name):
+    """The oracle's flights across the measurement boundaries (the engine's loop) against the
+    reference's loop as written, one drift per measurement step (modelSimulator.cpp:139-198), on
+    the same seed.  Both draw the same random numbers at the same events, so they agree up to the
+    few trajectories that rounding sends elsewhere: every counter within 2e-3.  This holds the
+    flights to the reference's semantics in each place they could differ: boundaries passed
+    during a flight must still count as steps and be tallied (tallies, phonon-steps), the step
+    of a collision must be that of its moment (the film's burst surfaces absorb only inside
+    their window), and stuck phonons must be relocated after 100 collisions in one step's drift,
+    not in one flight (the needles relocate 172 times either way at seed 0, 1465 times with
+    collisions counted per flight)."""
+    fix, _ = load_fixture(name) if name != "needles" else (json.load(open(os.path.join(GOLDEN, "ref_needles.json"))), None)
+    a = oracle.Model(fix["model"]).run(0, nthreads=8)
+    b = oracle.Model(fix["model"]).run(0, nthreads=8, step_by_step=True)
+    for key in ["phonon_steps", "scatters", "impacts", "tallies", "absorbed"]:
+        assert abs(a.counters[key] - b.counters[key]) <= 2e-3 * max(1, b.counters[key]), (key, a.counters, b.counters)
+    assert abs(a.counters["expired"] - b.counters["expired"]) <= max(20, 0.01 * b.counters["expired"])
+    assert abs(a.counters["relocations"] - b.counters["relocations"]) <= max(10, 0.1 * b.counters["relocations"])
+    assert a.counters["stalled"] == b.counters["stalled"] == 0
+
+
 STATS_ONLY = ["si_ge", "phasor", "needles"]
 
 
