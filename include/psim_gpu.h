@@ -208,7 +208,10 @@ int psim_gpu_fetch_fate(psim_gpu_ctx* ctx, int64_t* step, int32_t* cause, double
 int psim_gpu_last_timing(const psim_gpu_ctx* ctx, double* step_kernel_ms, int64_t* step_launches,
     double* run_ms);
 
-/* Engine tuning: measurement steps per transport launch (0 = automatic). */
+/* Engine tuning: measurement steps per transport launch (0 = automatic).  No length changes a
+ * result.  Recording launches (LDS tallies) are rounded down to a multiple of 8 steps, so that
+ * they hold whole flights (which end at every 8th recorded boundary); one of fewer than 8 steps
+ * lets flights pass its end, and those boundaries are tallied into HBM with atomics (slower). */
 int psim_gpu_set_window(psim_gpu_ctx* ctx, int32_t steps_per_launch);
 
 /* Engine tuning: transport blocks per compute unit (default 8). */

@@ -128,6 +128,8 @@ __device__ __forceinline__ Line make_line(double x1, double y1, double x2, doubl
     return l;
 }
 
+// An edge's line from its endpoints, slope and intercept.  The bounding box is the min/max of the
+// endpoints, as the host builds it (the same values as the record's box), so it is not loaded.
 __device__ __forceinline__ Line edge_line(const psim_edge& e) {
     Line l;
     l.p1x = e.p1x;
@@ -136,10 +138,10 @@ __device__ __forceinline__ Line edge_line(const psim_edge& e) {
     l.p2y = e.p2y;
     l.slope = e.slope;
     l.intercept = e.intercept;
-    l.blx = e.blx;
-    l.bly = e.bly;
-    l.trx = e.trx;
-    l.try_ = e.try_;
+    l.blx = (l.p2x <= l.p1x) ? l.p2x : l.p1x;
+    l.trx = (l.p2x <= l.p1x) ? l.p1x : l.p2x;
+    l.bly = (l.p2y <= l.p1y) ? l.p2y : l.p1y;
+    l.try_ = (l.p2y <= l.p1y) ? l.p1y : l.p2y;
     return l;
 }
 
