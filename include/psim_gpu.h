@@ -209,8 +209,8 @@ int psim_gpu_last_timing(const psim_gpu_ctx* ctx, double* step_kernel_ms, int64_
     double* run_ms);
 
 /* Engine tuning: measurement steps per transport launch (0 = automatic).  No length changes a
- * result.  Recording launches (LDS tallies) are rounded down to a multiple of 8 steps, so that
- * they hold whole flights (which end at every 8th recorded boundary); one of fewer than 8 steps
+ * result.  Recording launches (LDS tallies) are rounded down to a multiple of 16 steps, so that
+ * they hold whole flights (which end at every 16th recorded boundary); one of fewer than 16 steps
  * lets flights pass its end, and those boundaries are tallied into HBM with atomics (slower). */
 int psim_gpu_set_window(psim_gpu_ctx* ctx, int32_t steps_per_launch);
 

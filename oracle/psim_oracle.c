@@ -43,7 +43,7 @@
 /* Flights through the recorded steps also end at every FLIGHT_STEPS-th boundary (counted from
  * the first boundary of the recorded stretch), so that a window of a multiple of it holds whole
  * flights; the engine uses the same constant. */
-#define FLIGHT_STEPS 8
+#define FLIGHT_STEPS 16
 #define TEMP_BOUND_EPS 10.0
 #define PHASOR_TEMP_BOUND_EPS (TEMP_BOUND_EPS * 100.0)
 #define SS_STEPS_PERCENT 0.1

@@ -39,7 +39,8 @@ namespace {
 constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
 constexpr int LDS_TALLY_BYTES = 48 * 1024;  // budget for one block's [W][S] window tally
-constexpr int GLOBAL_TALLY_STEPS = 8;  // fewer LDS-tallied steps per window than this: global atomics
+constexpr int GLOBAL_TALLY_STEPS = FLIGHT_STEPS;  // fewer LDS-tallied steps per window than this (a window
+                                                   // that cannot hold whole flights): global atomics
 // Fewer phonons per sensor than this: global atomics too.  Small runs lose more to the many short
 // LDS windows (launches, reductions, sorts) than HBM atomics lose to contention, large ones the
 // reverse (measured at 1e4 and 1e5 phonons per sensor: 1.8-3x faster in HBM below, 2.7x slower
