@@ -421,6 +421,8 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": dict(desc, phonons=total),
+        "method": "phonons fly across the measurement boundaries; each boundary passed is counted as a phonon-step "
+                  "and, when recorded, tallied with the phonon's sensor and velocity of that moment",
         "roofline": {
             "bound": "hbm",
             "kernel": "transport_kernel",
