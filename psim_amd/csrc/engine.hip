@@ -993,7 +993,7 @@ constexpr int32_t DEFAULT_FREE_WINDOW = 1 << 30;
 
 int32_t record_window_for(const psim_gpu_ctx* c) {
     const int64_t by_lds = std::max<int64_t>(1, LDS_TALLY_BYTES / (20 * std::max(1, c->n_sensors)));
-    const int64_t w = std::min<int64_t>(c->window > 0 ? c->window : 64, by_lds);
+    const int64_t w = std::min<int64_t>(c->window > 0 ? c->window : 128, by_lds);
     // A multiple of FLIGHT_STEPS holds whole flights: no tally spills past the window into HBM.
     // Shorter windows let flights pass their end (same results, HBM atomics for those steps).
     return (int32_t)(w >= FLIGHT_STEPS ? w - w % FLIGHT_STEPS : w);
