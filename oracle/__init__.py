@@ -70,7 +70,8 @@ def _load():
     lib.or_math.argtypes = [ctypes.c_int, _D, _D, ctypes.c_int64]
     lib.or_philox.argtypes = [_U32, _U32, _U32]
     lib.or_philox.restype = None
-    lib.or_stream.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int64, _D]
+    lib.or_stream.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int64,
+                              ctypes.c_int, _D]
     lib.or_stream.restype = None
     lib.or_freq_index.argtypes = [_D, _D, ctypes.c_double, ctypes.c_double, _I64, _I32]
     lib.or_freq_index.restype = None
@@ -262,10 +263,11 @@ def philox(ctr, key) -> np.ndarray:
     return out
 
 
-def stream(seed: int, tag: int, ident: int, step: int, n: int) -> np.ndarray:
+def stream(seed: int, tag: int, ident: int, step: int, n: int, pairs: bool = False) -> np.ndarray:
+    """n single draws of a stream, or n / 2 pair draws (pairs=True)."""
     lib = _load()
     out = np.zeros(n, np.float64)
-    lib.or_stream(int(seed), int(tag), int(ident), int(step), int(n), _p(out, _D))
+    lib.or_stream(int(seed), int(tag), int(ident), int(step), int(n), int(bool(pairs)), _p(out, _D))
     return out
 
 

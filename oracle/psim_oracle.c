@@ -1062,8 +1062,8 @@ static void freq_index_draws(const table_t* t, double rand, double u2, int64_t* 
 }
 
 static void freq_index(const table_t* t, psim_stream* st, int64_t* index, int* polar) {
-    const double rand = psim_urand(st);
-    const double u2 = psim_urand(st);
+    double rand, u2; /* the bisection's draw and the branch draw, one block */
+    psim_urand2(st, &rand, &u2);
     freq_index_draws(t, rand, u2, index, polar);
 }
 
@@ -1082,15 +1082,18 @@ static void resample(const material_t* mt, const table_t* t, phonon_t* p, psim_s
 
 /* Phonon::setRandDirection, phonon.cpp:28-31 */
 static void set_rand_direction(phonon_t* p, psim_stream* st) {
-    p->dx = 2. * psim_urand(st) - 1.;
-    p->dy = sqrt(1. - p->dx * p->dx) * psim_cos2pi(psim_urand(st));
+    double u1, u2;
+    psim_urand2(st, &u1, &u2);
+    p->dx = 2. * u1 - 1.;
+    p->dy = sqrt(1. - p->dx * p->dx) * psim_cos2pi(u2);
 }
 
 /* Surface::redirectPhonon (Lambertian about the normal), surface.cpp:23-30 */
 static void redirect(pt n, phonon_t* p, psim_stream* st) {
-    const double rand = psim_urand(st);
+    double rand, u2;
+    psim_urand2(st, &rand, &u2);
     const double new_dx = sqrt(rand);
-    const double new_dy = sqrt(1. - rand) * psim_cos2pi(psim_urand(st));
+    const double new_dy = sqrt(1. - rand) * psim_cos2pi(u2);
     p->dx = n.x * new_dx - n.y * new_dy;
     p->dy = n.y * new_dx + n.x * new_dy;
 }
@@ -1247,8 +1250,8 @@ static int handle_impacts(const or_model* m, phonon_t* p, double drift_time, int
         if (p->cell < 0) { return 0; }
         drifted += t;
         if (++collisions > MAX_COLLISIONS) {
-            const double r1 = psim_urand(st);
-            const double r2 = psim_urand(st);
+            double r1, r2;
+            psim_urand2(st, &r1, &r2);
             const pt q = tri_rand_point(&m->cells[p->cell].tri, r1, r2); /* Phonon::setRandPoint */
             p->px = q.x;
             p->py = q.y;
@@ -1287,10 +1290,10 @@ static void transport_rates(const or_model* m, const phonon_t* p, double* r) {
     r[2] = mt->b_i * psim_pow4(freq);
 }
 
-/* ModelSimulator::scatter, modelSimulator.cpp:124-137 */
-static void scatter(const or_model* m, phonon_t* p, const double* r, psim_stream* st) {
+/* ModelSimulator::scatter, modelSimulator.cpp:124-137, with its draw `rand` taken by the caller
+ * (together with the next scattering-time draw, one block). */
+static void scatter(const or_model* m, phonon_t* p, const double* r, double rand, psim_stream* st) {
     const double tau_inv = 0. + r[0] + r[1] + r[2];
-    const double rand = psim_urand(st);
     if (rand <= (r[0] + r[1]) / tau_inv) {
         const sensor_t* s = &m->sensors[m->cells[p->cell].sensor];
         resample(&m->mats[s->mat], s->scatter, p, st);
@@ -1343,6 +1346,8 @@ static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint3
         double time_to_scatter = 0.;
         double time_to_measurement = 0.;
         int events = 0;
+        int scatter_pending = 0; /* the scattering-time draw after a scatter is next_draw */
+        double next_draw = 0.;
         const int64_t stop = m->step_adjustment >= 2 ? m->step_adjustment - 1 : 0;
 
         for (;;) {
@@ -1352,9 +1357,12 @@ static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint3
                 break;
             }
             if (time_to_scatter <= 0.) {
+                /* after a scatter, the second double of the scatter's block; else a block's first */
+                const double u = scatter_pending ? next_draw : psim_urand(&st);
                 transport_rates(m, p, relax);
-                time_to_scatter = SCALING_FACTOR * -psim_log(psim_urand(&st)) / (0. + relax[0] + relax[1] + relax[2]);
+                time_to_scatter = SCALING_FACTOR * -psim_log(u) / (0. + relax[0] + relax[1] + relax[2]);
             }
+            scatter_pending = 0;
             int64_t cap;
             if (m->step_by_step) { /* modelSimulator.cpp:159: the next boundary, kept until it is reached */
                 cap = step + 1;
@@ -1399,7 +1407,10 @@ static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint3
                 } else {
                     if (!m->phasor && time_to_scatter == 0.) {
                         ta->counters[OR_CNT_SCATTERS]++;
-                        scatter(m, p, relax, &st);
+                        double u;
+                        psim_urand2(&st, &u, &next_draw);
+                        scatter_pending = 1;
+                        scatter(m, p, relax, u, &st);
                     } else if (!relocated) { /* a new sensor: a new scattering time */
                         time_to_scatter = 0.;
                     }
@@ -1444,8 +1455,8 @@ static void build_phonon(const or_model* m, int64_t id, uint64_t seed, phonon_t*
         p->sign = (t_init > m->t_eq) ? 1 : -1;
         p->lifetime = 0.;
         resample(mt, s->base, p, &st);
-        const double r1 = psim_urand(&st);
-        const double r2 = psim_urand(&st);
+        double r1, r2;
+        psim_urand2(&st, &r1, &r2);
         const pt q = tri_rand_point(&c->tri, r1, r2);
         p->px = q.x;
         p->py = q.y;
@@ -1768,10 +1779,14 @@ void or_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
     for (int i = 0; i < 4; ++i) { out[i] = r.v[i]; }
 }
 
-void or_stream(uint64_t seed, uint32_t tag, uint32_t id, uint32_t step, int64_t n, double* out) {
+void or_stream(uint64_t seed, uint32_t tag, uint32_t id, uint32_t step, int64_t n, int pairs, double* out) {
     psim_stream st;
     psim_stream_init(&st, seed, tag, id, step);
-    for (int64_t i = 0; i < n; ++i) { out[i] = psim_urand(&st); }
+    if (pairs) {
+        for (int64_t i = 0; i + 1 < n; i += 2) { psim_urand2(&st, &out[i], &out[i + 1]); }
+    } else {
+        for (int64_t i = 0; i < n; ++i) { out[i] = psim_urand(&st); }
+    }
 }
 
 /* Material::freqIndex with the draws u1 (bisection) and u2 (polarization). */

@@ -104,7 +104,8 @@ int or_cell_geometry(const or_model* m, int cell, double* area, double* lines, d
  * direction sampling (0 isotropic, 1 Lambertian about n, 2 specular reflection of out[]). */
 int or_math(int fn, const double* x, double* y, int64_t n);
 void or_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
-void or_stream(uint64_t seed, uint32_t tag, uint32_t id, uint32_t step, int64_t n, double* out);
+/* n draws of a stream: single draws (psim_urand), or n / 2 pair draws (psim_urand2) */
+void or_stream(uint64_t seed, uint32_t tag, uint32_t id, uint32_t step, int64_t n, int pairs, double* out);
 void or_freq_index(const double* cdf, const double* frac, double u1, double u2, int64_t* bin, int32_t* pol);
 void or_directions(int kind, uint64_t seed, int64_t n, double nx, double ny, double* out);
 

@@ -115,8 +115,8 @@ __global__ void __launch_bounds__(BLOCK) emit_kernel(Model m, Pool pool, int64_t
         double birth = 0.;
         if (src.kind == 0) {  // CellOriginBuilder::operator()
             resample(m, mat, src.table, p, st);
-            double r1 = psim_urand(&st);
-            double r2 = psim_urand(&st);
+            double r1, r2;
+            psim_urand2(&st, &r1, &r2);
             if (r1 + r2 > 1.) {
                 r1 = 1 - r1;
                 r2 = 1 - r2;
@@ -570,8 +570,8 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                     cur_sensor = m.cells[p.cell].sensor;
                     drifted += t_hit;
                     if (++collisions > MAX_COLLISIONS) {  // Phonon::setRandPoint -> Triangle::getRandPoint
-                        double r1 = psim_urand(&st);
-                        double r2 = psim_urand(&st);
+                        double r1, r2;
+                        psim_urand2(&st, &r1, &r2);
                         const psim_cell& cl = m.cells[p.cell];
                         if (r1 + r2 > 1.) {
                             r1 = 1 - r1;
@@ -595,9 +595,15 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                 }
             }
         } else {  // S_SCAT: scatter (lanes in S_SCAT), then the scattering-time draw
+            // One block for every lane of the stage: a scattering lane takes the scatter's draw
+            // and its next scattering-time draw from it, a lane that only draws a scattering time
+            // takes the first double.
+            double u_a = 0., u_b = 0.;
+            if (state == S_SCAT || state == S_TTS) { psim_urand2(&st, &u_a, &u_b); }
             if (state == S_SCAT) {
                 c.scatters++;
-                scatter(m, p, relax, st);
+                scatter(m, p, relax, u_a, st);
+                u_a = u_b;
                 if (++events > PSIM_MAX_EVENTS_PER_STEP) {  // begin_event's stall bound
                     c.stalled++;
                     state = S_DEAD + FATE_STALLED * 8;
@@ -607,7 +613,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
             }
             if (state == S_TTS) {
                 relax_rates(m, cur_sensor, p, relax);
-                tts = SCALING_FACTOR * -psim_log(psim_urand(&st)) / (0. + relax[0] + relax[1] + relax[2]);
+                tts = SCALING_FACTOR * -psim_log(u_a) / (0. + relax[0] + relax[1] + relax[2]);
                 drift_setup();
             }
         }

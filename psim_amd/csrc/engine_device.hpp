@@ -208,7 +208,8 @@ __device__ __forceinline__ bool get_intersection(const Line& s, const Line& o, d
 // have no guide and run the bisection.
 __device__ __forceinline__ void freq_index(const double* __restrict__ cdf, const double* __restrict__ frac,
     const uint16_t* __restrict__ guide, psim_stream& st, int& bin, int& pol) {
-    const double rand = psim_urand(&st);
+    double rand, upol;  // the bisection's draw and the branch draw, one block
+    psim_urand2(&st, &rand, &upol);
     int high;
     if (guide) {
         const int g = (int)(rand * (double)GUIDE);
@@ -230,7 +231,7 @@ __device__ __forceinline__ void freq_index(const double* __restrict__ cdf, const
         }
     }
     bin = high;
-    pol = (psim_urand(&st) <= frac[high]) ? POL_LA : POL_TA;
+    pol = (upol <= frac[high]) ? POL_LA : POL_TA;
 }
 
 __device__ __forceinline__ double bin_velocity(const Model& m, int mat, int bin, int pol) {  // material.cpp:82-84
@@ -252,14 +253,17 @@ __device__ __forceinline__ void resample(const Model& m, int mat, int table, Ph&
 }
 
 __device__ __forceinline__ void set_rand_direction(Ph& p, psim_stream& st) {  // phonon.cpp:28-31
-    p.dx = 2. * psim_urand(&st) - 1.;
-    p.dy = sqrt(1. - p.dx * p.dx) * psim_cos2pi(psim_urand(&st));
+    double u1, u2;
+    psim_urand2(&st, &u1, &u2);
+    p.dx = 2. * u1 - 1.;
+    p.dy = sqrt(1. - p.dx * p.dx) * psim_cos2pi(u2);
 }
 
 __device__ __forceinline__ void redirect(double nx, double ny, Ph& p, psim_stream& st) {  // surface.cpp:23-30
-    const double rand = psim_urand(&st);
+    double rand, u2;
+    psim_urand2(&st, &rand, &u2);
     const double new_dx = sqrt(rand);
-    const double new_dy = sqrt(1. - rand) * psim_cos2pi(psim_urand(&st));
+    const double new_dy = sqrt(1. - rand) * psim_cos2pi(u2);
     p.dx = nx * new_dx - ny * new_dy;
     p.dy = ny * new_dx + nx * new_dy;
 }
@@ -370,10 +374,10 @@ __device__ __forceinline__ void relax_rates(const Model& m, int sensor, const Ph
     r[2] = mt.b_i * psim_pow4(freq);
 }
 
-// ModelSimulator::scatter, modelSimulator.cpp:124-137
-__device__ __forceinline__ void scatter(const Model& m, Ph& p, const double* r, psim_stream& st) {
+// ModelSimulator::scatter, modelSimulator.cpp:124-137, with its draw `rand` taken by the caller
+// (together with the next scattering-time draw, one block).
+__device__ __forceinline__ void scatter(const Model& m, Ph& p, const double* r, double rand, psim_stream& st) {
     const double tau_inv = 0. + r[0] + r[1] + r[2];
-    const double rand = psim_urand(&st);
     if (rand <= (r[0] + r[1]) / tau_inv) {
         const psim_sensor& s = m.sensors[m.cells[p.cell].sensor];
         resample(m, s.material, s.scatter_table, p, st);

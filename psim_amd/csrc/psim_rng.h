@@ -15,8 +15,12 @@
  *   c1 = measurement step for PSIM_TAG_TRANSPORT, 0 otherwise
  *   c2 = phonon id (transport, emission) or source index (budget)
  *   c3 = stream tag | stream index << 8 (the index is 0 except for batched models)
- * Draw n of a stream is the (n % 2)-th double of block n / 2.  A transport
- * stream restarts at draw 0 whenever the phonon's measurement step advances.
+ * Every draw takes the stream's next block: psim_urand returns its first double,
+ * psim_urand2 both, for two numbers used together.  (A spare second double kept
+ * per lane would leave the lanes of a wave out of step, so that nearly every
+ * single draw would run a Philox block for the wave anyway; fixed pairs keep the
+ * blocks a wave runs to one per draw site.)  A transport stream restarts at
+ * block 0 whenever the phonon's measurement step advances.
  *
  * Plain C99 so the oracle (gcc) and the HIP kernels (hipcc) share it.
  */
@@ -74,13 +78,11 @@ PSIM_QUAL double psim_u01_from_pair(uint32_t hi, uint32_t lo) {
     return (double)(bits >> 11) * 0x1.0p-53;
 }
 
-/* A stream of uniform doubles in [0, 1): two per Philox block. */
+/* A stream of uniform doubles in [0, 1): one Philox block per draw. */
 typedef struct {
     uint32_t k0, k1; /* key = seed */
     uint32_t c1, c2, c3; /* fixed counter words */
     uint32_t block; /* next block index */
-    double spare; /* second double of the last block */
-    int has_spare;
 } psim_stream;
 
 PSIM_QUAL void psim_stream_init(psim_stream* s, uint64_t seed, uint32_t tag, uint32_t id, uint32_t step) {
@@ -90,8 +92,6 @@ PSIM_QUAL void psim_stream_init(psim_stream* s, uint64_t seed, uint32_t tag, uin
     s->c2 = id;
     s->c3 = tag;
     s->block = 0;
-    s->spare = 0.0;
-    s->has_spare = 0;
 }
 
 /* Counter word c3 of a stream: its tag plus a stream index (a model's index in a batch). */
@@ -101,23 +101,28 @@ PSIM_QUAL uint32_t psim_stream_tag(uint32_t tag, uint32_t stream) { return tag |
 PSIM_QUAL void psim_stream_set_step(psim_stream* s, uint32_t step) {
     s->c1 = step;
     s->block = 0;
-    s->has_spare = 0;
 }
 
-PSIM_QUAL double psim_urand(psim_stream* s) {
-    if (s->has_spare) {
-        s->has_spare = 0;
-        return s->spare;
-    }
+PSIM_QUAL psim_u32x4 psim_stream_block(psim_stream* s) {
     psim_u32x4 c;
     c.v[0] = s->block++;
     c.v[1] = s->c1;
     c.v[2] = s->c2;
     c.v[3] = s->c3;
-    const psim_u32x4 r = psim_philox4x32_10(c, s->k0, s->k1);
-    s->spare = psim_u01_from_pair(r.v[2], r.v[3]);
-    s->has_spare = 1;
+    return psim_philox4x32_10(c, s->k0, s->k1);
+}
+
+/* One uniform double: the first of the next block. */
+PSIM_QUAL double psim_urand(psim_stream* s) {
+    const psim_u32x4 r = psim_stream_block(s);
     return psim_u01_from_pair(r.v[0], r.v[1]);
+}
+
+/* Two uniform doubles from the next block, in order. */
+PSIM_QUAL void psim_urand2(psim_stream* s, double* a, double* b) {
+    const psim_u32x4 r = psim_stream_block(s);
+    *a = psim_u01_from_pair(r.v[0], r.v[1]);
+    *b = psim_u01_from_pair(r.v[2], r.v[3]);
 }
 
 #endif /* PSIM_RNG_H */
