@@ -91,6 +91,26 @@ def test_trajectories_match_oracle(engine, name, tally):
     np.testing.assert_allclose(res.measurements[:, 2:], meas[:, 2:], rtol=1e-9, atol=1e-9 * np.max(np.abs(meas[:, 2:])))
 
 
+@pytest.mark.parametrize("sensors,length,launches", [(10, 1000.0, 8), (40, 4000.0, 21)])
+def test_recording_windows_hold_whole_flights(engine, sensors, length, launches):
+    """Default LDS recording windows are sized by the 48 KB budget (20 B per sensor and step), at
+    most 128 steps, and rounded down to a multiple of the 16-step flight cap, so no flight passes
+    a window's end: 10 sensors give 128-step windows (1000 steps in 8 launches), 40 sensors 61
+    rounded to 48 (21 launches)."""
+    cfg = configs.linear_bar(num_sensors=sensors, length=length, num_phonons=20000, num_measurements=1000,
+                             sim_time=5.0, spec=0.5, sim_type=1, step_interval=50).to_dict()
+    print(f"case {sensors} sensors", flush=True)
+    model = psim_amd.Model(cfg)
+    engine.set_fate(False)
+    engine.set_window(0)
+    engine.set_tally_mode("lds")
+    try:
+        model.run(engine, 3)
+        assert engine.last_timing()["step_launches"] == launches
+    finally:
+        engine.set_tally_mode("auto")
+
+
 @pytest.mark.parametrize("window", [1, 7, 333])
 @pytest.mark.parametrize("name", ["bar_periodic_mixed", "bar_ss_diffuse"])
 def test_launch_window_does_not_change_results(engine, name, window):
