@@ -49,6 +49,8 @@ def _load():
     lib.or_add_emit.argtypes = [vp, _D]
     lib.or_prepare.argtypes = [vp, ctypes.c_uint64]
     lib.or_set_step_by_step.argtypes = [vp, ctypes.c_int]
+    lib.or_set_stream.argtypes = [vp, ctypes.c_int]
+    lib.or_plant_flight_fault.argtypes = [vp, ctypes.c_int]
     lib.or_num_sensors.argtypes = [vp]
     lib.or_num_record.restype = ctypes.c_int64
     lib.or_num_record.argtypes = [vp]
@@ -145,6 +147,16 @@ class Model:
         if h:
             self._lib.or_model_free(h)
             self._h = None
+
+    def set_stream(self, stream: int) -> None:
+        """Random-stream index of every draw: stream m reproduces model m of a batched run."""
+        self._check(self._lib.or_set_stream(self._h, int(stream)))
+        self.seed = None
+
+    def plant_flight_fault(self, kind: int) -> None:
+        """Test hook: a known mistake in the tallies of the boundaries a flight passes (1: one index
+        late, 2: into the next sensor, 0: none)."""
+        self._check(self._lib.or_plant_flight_fault(self._h, int(kind)))
 
     def prepare(self, seed: int) -> None:
         self._check(self._lib.or_prepare(self._h, int(seed)))

@@ -420,6 +420,8 @@ struct or_model {
     double step_time;
     int64_t step_adjustment; /* ModelSimulator::step_adjustment_ */
     int step_by_step; /* the reference's loop as written: one drift per measurement step */
+    uint32_t stream; /* random-stream index: model m of a batch draws the numbers of stream m */
+    int planted_fault; /* test hook (or_plant_flight_fault): 0, or a known mistake in the flights */
     int64_t start_step; /* Model::start_step_ */
     int64_t n_record; /* Sensor tally length */
     int n_mat, cap_mat;
@@ -913,6 +915,24 @@ int or_set_step_by_step(or_model* m, int on) {
     return 0;
 }
 
+/* Test hook: plants a known mistake in the tallies of the boundaries a flight passes, so that a
+ * test can show that its comparison detects it.  1: tallied one index late; 2: tallied into the
+ * next sensor.  0 removes it. */
+int or_plant_flight_fault(or_model* m, int kind) {
+    if (kind < 0 || kind > 2) { return set_err(m, "unknown planted fault"); }
+    m->planted_fault = kind;
+    return 0;
+}
+
+/* The random-stream index of every draw (budget, emission, transport), as a batch of models gives
+ * model m stream m with its phonon ids counted from 0; the budget must be prepared again. */
+int or_set_stream(or_model* m, int stream) {
+    if (stream < 0 || stream > 0xffff) { return set_err(m, "stream index out of range"); }
+    m->stream = (uint32_t)stream;
+    m->prepared = 0;
+    return 0;
+}
+
 int or_prepare(or_model* m, uint64_t seed) {
     if (m->n_cells == 0 || m->n_sensors == 0) { return set_err(m, "model has no cells or sensors"); }
     /* Model::setTemperatureBounds, model.cpp:203-218 */
@@ -989,7 +1009,7 @@ int or_prepare(or_model* m, uint64_t seed) {
             const double frac = modf(energy / m->eff_energy, &ipart);
             int64_t n = (int64_t)ipart;
             psim_stream st;
-            psim_stream_init(&st, seed, PSIM_TAG_BUDGET, sidx++, 0);
+            psim_stream_init(&st, seed, psim_stream_tag(PSIM_TAG_BUDGET, m->stream), sidx++, 0);
             if (psim_urand(&st) < frac) { ++n; }
             source_t* src = &m->sources[m->n_sources++];
             src->cell = ci;
@@ -1204,6 +1224,7 @@ typedef struct {
 } flight_t;
 
 static void tally(const or_model* m, const phonon_t* p, int64_t idx, tally_t* ta);
+static void tally_at(const or_model* m, const phonon_t* p, int s, int64_t idx, tally_t* ta);
 
 /* The measurement events of the boundaries a flight passed up to time t
  * (modelSimulator.cpp:182-188): *step advances to the last boundary k < cap whose time
@@ -1221,7 +1242,13 @@ static int advance_step(const or_model* m, phonon_t* p, int64_t* step, int64_t c
     while (k + 1 < cap && (double)(k + 1) * m->sim_time / (double)N <= t) { ++k; }
     if (k == *step) { return 0; }
     for (int64_t j = (*step + 1 > m->step_adjustment ? *step + 1 : m->step_adjustment); j <= k; ++j) {
-        tally(m, p, j - m->step_adjustment, ta);
+        if (m->planted_fault == 1) { /* test hook: one index late */
+            tally(m, p, j - m->step_adjustment + 1, ta);
+        } else if (m->planted_fault == 2) { /* test hook: the next sensor */
+            tally_at(m, p, (m->cells[p->cell].sensor + 1) % m->n_sensors, j - m->step_adjustment, ta);
+        } else {
+            tally(m, p, j - m->step_adjustment, ta);
+        }
     }
     ta->counters[OR_CNT_PHONON_STEPS] += k - *step;
     *step = k;
@@ -1304,9 +1331,8 @@ static void scatter(const or_model* m, phonon_t* p, const double* r, double rand
 }
 
 /* Sensor::updateHeatParams, sensor.cpp:43-52 */
-static void tally(const or_model* m, const phonon_t* p, int64_t idx, tally_t* ta) {
+static void tally_at(const or_model* m, const phonon_t* p, int s, int64_t idx, tally_t* ta) {
     if (idx >= m->n_record) { return; }
-    const int s = m->cells[p->cell].sensor;
     const double vx = p->dx * p->velocity;
     const double vy = p->dy * p->velocity;
     const int64_t k = (int64_t)s * m->n_record + idx;
@@ -1314,6 +1340,10 @@ static void tally(const or_model* m, const phonon_t* p, int64_t idx, tally_t* ta
     ta->flux[2 * k] += vx * (double)p->sign;
     ta->flux[2 * k + 1] += vy * (double)p->sign;
     ta->counters[OR_CNT_TALLIES]++;
+}
+
+static void tally(const or_model* m, const phonon_t* p, int64_t idx, tally_t* ta) {
+    tally_at(m, p, m->cells[p->cell].sensor, idx, ta);
 }
 
 /* ModelSimulator::simulatePhonon, modelSimulator.cpp:139-198, with flights in place of one
@@ -1341,7 +1371,7 @@ static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint3
     } else {
         p->lifestep = step;
         psim_stream st;
-        psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, id, (uint32_t)step);
+        psim_stream_init(&st, seed, psim_stream_tag(PSIM_TAG_TRANSPORT, m->stream), id, (uint32_t)step);
         double relax[3] = { 0, 0, 0 };
         double time_to_scatter = 0.;
         double time_to_measurement = 0.;
@@ -1447,7 +1477,7 @@ static void build_phonon(const or_model* m, int64_t id, uint64_t seed, phonon_t*
     const sensor_t* s = &m->sensors[c->sensor];
     material_t* mt = &m->mats[s->mat];
     psim_stream st;
-    psim_stream_init(&st, seed, PSIM_TAG_EMIT, (uint32_t)id, 0);
+    psim_stream_init(&st, seed, psim_stream_tag(PSIM_TAG_EMIT, m->stream), (uint32_t)id, 0);
     memset(p, 0, sizeof *p);
     p->cell = src->cell;
     if (src->kind == 0) { /* CellOriginBuilder::operator() */

@@ -47,6 +47,12 @@ int or_prepare(or_model* m, uint64_t seed);
 /* on != 0: run the reference's transport loop as written, one drift per measurement step,
  * instead of flights across the boundaries (a check of the flights; never used otherwise). */
 int or_set_step_by_step(or_model* m, int on);
+/* Random-stream index (0..65535) of every draw: stream m reproduces model m of a batched run,
+ * whose phonon ids count from 0 within the model.  Clears the prepared budget. */
+int or_set_stream(or_model* m, int stream);
+/* Test hook: plants a known mistake in the tallies of the boundaries a flight passes (1: one
+ * index late, 2: into the next sensor; 0: none), so that a test can show its check detects it. */
+int or_plant_flight_fault(or_model* m, int kind);
 
 int or_num_sensors(const or_model* m);
 int64_t or_num_record(const or_model* m);

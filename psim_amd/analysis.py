@@ -48,8 +48,13 @@ def time_to_steady_state(temps, window: int = 100, tol: float = 0.05, consecutiv
     return None
 
 
-def three_sigma(a_runs, b_mean, b_sd, b_runs: int) -> np.ndarray:
-    """z = |mean(a) - b_mean| / sqrt(sd(a)^2 / n_a + b_sd^2 / n_b) per entry (a: [runs, ...])."""
+def three_sigma(a_runs, b_mean, b_sd, b_runs: int, atol: float = 0.0) -> np.ndarray:
+    """z = |mean(a) - b_mean| / sqrt(sd(a)^2 / n_a + b_sd^2 / n_b) per entry (a: [runs, ...]).
+
+    Where neither side varies (standard error 0) there is no scale: z is 0 when the means agree
+    within `atol` and infinite otherwise, so a constant error cannot pass."""
     a = np.asarray(a_runs, dtype=np.float64)
-    se = np.sqrt(a.std(axis=0, ddof=1) ** 2 / a.shape[0] + np.asarray(b_sd) ** 2 / b_runs)
-    return np.abs(a.mean(axis=0) - np.asarray(b_mean)) / np.where(se > 0, se, np.inf)
+    se = np.sqrt(a.std(axis=0, ddof=1) ** 2 / a.shape[0] + np.asarray(b_sd, dtype=np.float64) ** 2 / b_runs)
+    delta = np.abs(a.mean(axis=0) - np.asarray(b_mean, dtype=np.float64))
+    flat = np.where(delta <= atol, 0.0, np.inf)
+    return np.where(se > 0, delta / np.where(se > 0, se, 1.0), flat)

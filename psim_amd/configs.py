@@ -284,6 +284,21 @@ def geometry_from_params(p: dict, num_measurements: int = 2000, sim_time: float 
     return b.to_dict()
 
 
+def kinked_structure(theta_deg: float, n_sensors: int = 21, length: float = 2000.0, width: float = 200.0,
+                     t_hot: float = 320.0, t_cold: float = 280.0, spec: float = 0.5, num_phonons: int = 100_000,
+                     num_measurements: int = 2000, sim_time: float = 10.0) -> dict:
+    """A kinked structure of the batched-geometry family with fixed parameters: two 1000 x 200 nm
+    arms, the second turned by `theta_deg` about the inner corner and joined to the first by a
+    wedge sensor, 10 sensors per arm, walls 320 / 280 K (deviational about 300 K), specularity
+    0.5, 1e5 phonons, 2000 steps of 5 ps.  Its second arm's edges lie at arbitrary slopes, the
+    general branch of Line::getIntersection (psim/src/geometry.cpp:104-138)."""
+    import math
+
+    p = {"kinked": True, "n_sensors": n_sensors, "length": length, "width": width, "t_hot": t_hot,
+         "t_cold": t_cold, "spec": spec, "phonons": num_phonons, "theta": math.radians(theta_deg)}
+    return geometry_from_params(p, num_measurements, sim_time)
+
+
 def batch_geometry(rng, num_measurements: int = 2000, sim_time: float = 10.0) -> dict:
     """One structure of the batched-geometry workload (batch_geometry_params, geometry_from_params)."""
     return geometry_from_params(batch_geometry_params(rng), num_measurements, sim_time)

@@ -84,6 +84,14 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+// The wave's sum of per-lane 32-bit counters, in 64 bits: 64 lanes of a long launch can together
+// pass 2^32 phonon-steps.
+__device__ __forceinline__ unsigned long long wave_sum64(uint32_t v) {
+    unsigned long long x = v;
+    for (int off = WAVE / 2; off > 0; off >>= 1) { x += __shfl_xor(x, off, WAVE); }
+    return x;
+}
+
 // Phonon builders, psim/src/phononBuilder.cpp:6-49, with EmitSurface::getPhononTime
 // (surface.cpp:67-69) and Line/Triangle::getRandPoint (geometry.cpp:140-143, 234-242).
 __global__ void __launch_bounds__(BLOCK) emit_kernel(Model m, Pool pool, int64_t total,
@@ -678,8 +686,8 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     const int slots[8] = { PSIM_CNT_PHONON_STEPS, PSIM_CNT_SCATTERS, PSIM_CNT_IMPACTS, PSIM_CNT_RELOCATIONS,
         PSIM_CNT_ABSORBED, PSIM_CNT_EXPIRED, PSIM_CNT_TALLIES, PSIM_CNT_STALLED };
     for (int i = 0; i < 8; ++i) {
-        const uint32_t sum = wave_sum(v[i]);
-        if (lane == 0 && sum) { atomicAdd(&counters[slots[i]], (unsigned long long)sum); }
+        const unsigned long long sum = wave_sum64(v[i]);
+        if (lane == 0 && sum) { atomicAdd(&counters[slots[i]], sum); }
     }
     if (WS) {
         __syncthreads();
@@ -1095,6 +1103,14 @@ int prepare_run(psim_gpu_ctx* c, RunPlan& plan) {
     }
     const int64_t blocks_needed = (n + BLOCK - 1) / BLOCK;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks_needed, (int64_t)c->n_cu * c->blocks_per_cu));
+    // Each lane counts its phonon-steps, tallies and events in 32 bits until the launch ends (the
+    // wave sums are 64-bit).  The work queue spreads the phonons evenly over the lanes, so a lane
+    // carries about n * N / lanes phonon-steps at most; runs where that average comes within a
+    // factor of 2 of 2^32 are refused rather than risk a wrapped counter.
+    const double lane_share = (double)n * (double)P.num_measurements / ((double)grid * BLOCK);
+    if (lane_share >= 2147483648.0) {
+        return fail(c, PSIM_E_INVALID, "run too large: phonons x measurement steps per lane exceeds 2^31");
+    }
     // A live buffer holds every phonon plus the survivor slots the waves may reserve but not fill.
     const int64_t live_need = n + (int64_t)OUT_CLAIM * grid * (BLOCK / WAVE);
     if (c->list_cap < live_need) {

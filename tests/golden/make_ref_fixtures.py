@@ -60,6 +60,12 @@ def workloads():
         "si_ge": (configs.si_ge_bar(num_phonons=100_000, num_measurements=2000, sim_time=10.0), None),
         "phasor": (_phasor_bar(), None),
         "needles": (configs.needle_bar(num_phonons=100_000, num_measurements=2000, sim_time=10.0), None),
+        # the linear_ss bar above the reference's PHONON_CUTOFF of 5e6 (modelSimulator.cpp:18), so
+        # that its runs take runUsingBuilders (:245-253) instead of runPhononByPhonon
+        "linear_b": (configs.linear_ss(num_phonons=6_000_000), None),
+        # kinked structures of the batched-geometry family: arms at 15 and 90 degrees
+        "kinked_15": (configs.kinked_structure(15.0), None),
+        "kinked_90": (configs.kinked_structure(90.0), None),
     }
 
 

@@ -55,6 +55,17 @@ def test_three_sigma_scaling():
     np.testing.assert_allclose(z, 3.0)
 
 
+def test_three_sigma_without_spread():
+    """Entries where neither side varies: equal means score 0, a constant offset fails, and an
+    offset within atol passes."""
+    a = np.full((16, 3), 300.0)
+    a[:, 2] += np.linspace(-1.0, 1.0, 16)  # the third entry varies
+    z = analysis.three_sigma(a, [300.0, 300.5, 300.0], [0.0, 0.0, 0.0], 16)
+    assert z[0] == 0.0 and np.isinf(z[1]) and np.isfinite(z[2])
+    z = analysis.three_sigma(a, [300.0, 300.005, 300.0], [0.0, 0.0, 0.0], 16, atol=0.01)
+    assert z[1] == 0.0
+
+
 def test_k_eff_fixture_is_consistent():
     fix = json.load(open(os.path.join(HERE, "golden", "ref_linear_k.json")))
     per = np.array(fix["per_run"]["x_flux"])

@@ -2,14 +2,19 @@
 
 The reference's RNG cannot be seeded, so agreement is statistical: over 16 GPU seeds and 16
 reference runs every sensor's mean temperature and x/y flux must agree within 3 standard errors
-of the difference, |mean_gpu - mean_ref| <= 3 sqrt(s_gpu^2/16 + s_ref^2/16), with the bound
-applied to the sensors' entries as a family (tests/family.py: with standard errors estimated
-from 16 runs each z follows Student's t, so the number of entries beyond 3 may not exceed the
-99.9th percentile of Binomial(n, P(|t| > 3)) and the largest z may not exceed the bound a
-family of n entries passes with probability 99.9 %).  The transient film is also compared block
-by block (100-step block means, the periodic export's series) with the same family rule.  The same film with the
-benchmark's 1e8-phonon pool is compared with the reference means over the fixture's 2000 steps.  The effective
-conductivity of the 1000 nm bar (1e6 phonons, 48 runs on each side) must agree within 1 %.
+of the difference, |mean_gpu - mean_ref| <= 3 sqrt(s_gpu^2/16 + s_ref^2/16), entry by entry.
+
+  - linear_ss, lshape_r, film_r: the three workloads at the fixtures' sizes;
+  - lshape at 1e7 phonons: the benchmark's own size, 16 GPU seeds against the reduced reference
+    runs (the phonon count changes the noise, not the expectation);
+  - linear_b: the bar at 6e6 phonons, above the reference's PHONON_CUTOFF (5e6), where the
+    reference simulates builder by builder (runUsingBuilders, modelSimulator.cpp:245-253);
+  - the transient film block by block (600 100-step block means, the periodic export's series):
+    a family of that size is judged by tests/family.py (the number of entries beyond 3 within the
+    99.9th percentile of Binomial(n, P(|t| > 3)) and the largest z within the 99.9 % family
+    bound), since z follows Student's t and a fixed cut on 600 entries fails by chance;
+  - the same film with the benchmark's 1e8-phonon pool against the reference means;
+  - the effective conductivity of the 1000 nm bar (1e6 phonons, 48 runs on each side) within 1 %.
 """
 import json
 import os
@@ -48,26 +53,55 @@ def _gpu_runs(engine, model, seeds):
     return out
 
 
-@pytest.mark.parametrize("name", ["linear_ss", "lshape_r", "film_r"])
-def test_sensor_means_within_three_sigma(engine, name):
-    fix = _fixture(name)
-    print(f"case {name}: {fix['runs']} seeds", flush=True)
-    runs = _gpu_runs(engine, fix["model"], range(fix["runs"]))
+def _sensor_z(runs, fix):
     zs = []
     for col, key in [(0, "t_steady"), (2, "x_flux"), (4, "y_flux")]:
         z = analysis.three_sigma([r.measurements[:, col] for r in runs], fix["mean"][key], fix["sd"][key],
                                  fix["runs"])
         print(key, np.round(z, 2), flush=True)
         zs.append(z)
-    ok, detail = family_ok(np.concatenate(zs), 2 * (fix["runs"] - 1))
-    print("sensor family (n, over 3, allowed, max z, bound):", detail, flush=True)
-    assert ok, detail
+    return np.concatenate(zs)
+
+
+@pytest.mark.parametrize("name", ["linear_ss", "lshape_r", "film_r"])
+def test_sensor_means_within_three_sigma(engine, name):
+    fix = _fixture(name)
+    print(f"case {name}: {fix['runs']} seeds", flush=True)
+    runs = _gpu_runs(engine, fix["model"], range(fix["runs"]))
+    z = _sensor_z(runs, fix)
+    assert z.max() <= 3.0, np.round(z, 2)
     if "block_mean" in fix:
         blocks = [analysis.block_means(r.temps, fix["block_interval"]) for r in runs]
         z = analysis.three_sigma(blocks, fix["block_mean"], fix["block_sd"], fix["runs"])
         ok, detail = family_ok(z, 2 * (fix["runs"] - 1))
         print("block family (n, over 3, allowed, max z, bound):", detail, flush=True)
         assert ok, detail
+
+
+def test_lshape_at_benchmark_size(engine):
+    """The benchmark workload itself: the L-shape at 1e7 phonons (5000 steps of 2 ps, full mode),
+    16 GPU seeds against the reference's 16 runs at 1e5 phonons.  More phonons only shrink the
+    spread, so the sensors' expected temperatures and fluxes are the same."""
+    from psim_amd import configs
+
+    fix = _fixture("lshape_r")
+    cfg = configs.lshape_full(num_phonons=10_000_000)
+    print("lshape at 1e7 phonons, 16 seeds", flush=True)
+    runs = _gpu_runs(engine, cfg, range(16))
+    z = _sensor_z(runs, fix)
+    assert z.max() <= 3.0, np.round(z, 2)
+
+
+def test_builder_path_of_the_reference(engine):
+    """The bar at 6e6 phonons, where the reference simulates builder by builder
+    (runUsingBuilders, modelSimulator.cpp:245-253) instead of one shuffled list of phonons: 16
+    GPU seeds against the reference's 16 runs of that size."""
+    fix = _fixture("linear_b")
+    assert fix["model"]["settings"]["num_phonons"] > 5_000_000
+    print(f"linear_b: {fix['runs']} seeds of 6e6 phonons", flush=True)
+    runs = _gpu_runs(engine, fix["model"], range(fix["runs"]))
+    z = _sensor_z(runs, fix)
+    assert z.max() <= 3.0, np.round(z, 2)
 
 
 def test_effective_conductivity_within_one_percent(engine):

@@ -10,7 +10,8 @@ What fixes each part of the oracle, independently of the oracle's own formulas:
                     exceeds the draw, clamped to [1, 999] (bin 0 folds into bin 1)
   directions        moments of the isotropic and Lambertian laws; exact specular reflection
   budget            counts are floor(E / E_eff) or one more, summing to the total
-  whole loop        16 seeds against 16 independent reference runs, 3 sigma per sensor
+  whole loop        16 seeds against 16 independent reference runs, every sensor's entries
+                    within 3 standard errors
 """
 import json
 import math
@@ -23,7 +24,7 @@ import pytest
 
 import oracle
 from family import family_ok
-from psim_amd import configs
+from psim_amd import analysis, configs
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -286,12 +287,7 @@ def test_budget_rounding():
 # ---- whole loop: 16 seeds against 16 reference runs ----------------------------------------------
 
 def _three_sigma(fix, ours, key):
-    ref_m, ref_s = np.array(fix["mean"][key]), np.array(fix["sd"][key])
-    mine = np.array([o[key] for o in ours])
-    m_m, m_s = mine.mean(axis=0), mine.std(axis=0, ddof=1)
-    se = np.sqrt(m_s**2 / len(mine) + ref_s**2 / fix["runs"])
-    z = np.abs(m_m - ref_m) / np.where(se > 0, se, np.inf)
-    return z
+    return analysis.three_sigma([o[key] for o in ours], fix["mean"][key], fix["sd"][key], fix["runs"])
 
 
 @pytest.mark.parametrize("name", WORKLOADS)
@@ -304,8 +300,7 @@ def test_oracle_statistics_match_reference(name):
         out, temps, _ = m.interpret(r.counts, r.flux)
         ours.append({"t_steady": out[:, 0], "x_flux": out[:, 2], "y_flux": out[:, 4]})
     zs = [_three_sigma(fix, ours, key) for key in ["t_steady", "x_flux", "y_flux"]]
-    ok, detail = family_ok(np.concatenate(zs), 2 * (fix["runs"] - 1))
-    assert ok, (detail, [np.round(z, 2) for z in zs])
+    assert max(float(z.max()) for z in zs) <= 3.0, [np.round(z, 2) for z in zs]
 
 
 @pytest.mark.parametrize("name", ["lshape_r", "film_r", "needles", "linear_ss"])
@@ -319,7 +314,7 @@ def test_flights_agree_with_the_step_by_step_loop(name):
     of a collision must be that of its moment (the film's burst surfaces absorb only inside
     their window), and stuck phonons must be relocated after 100 collisions in one step's drift,
     not in one flight (the needles relocate 172 times either way at seed 0, 1465 times with
-    collisions counted per flight)."""
+    collisions counted per flight).  The per-sensor, per-step tallies must agree as well."""
     fix, _ = load_fixture(name) if name != "needles" else (json.load(open(os.path.join(GOLDEN, "ref_needles.json"))), None)
     a = oracle.Model(fix["model"]).run(0, nthreads=8)
     b = oracle.Model(fix["model"]).run(0, nthreads=8, step_by_step=True)
@@ -328,9 +323,27 @@ def test_flights_agree_with_the_step_by_step_loop(name):
     assert abs(a.counters["expired"] - b.counters["expired"]) <= max(20, 0.01 * b.counters["expired"])
     assert abs(a.counters["relocations"] - b.counters["relocations"]) <= max(10, 0.1 * b.counters["relocations"])
     assert a.counters["stalled"] == b.counters["stalled"] == 0
+    # Where the tallies land: the per-sensor, per-step counts of the two loops.  The trajectories
+    # that rounding sends elsewhere move at most 0.4 % of the absolute tally mass (the L-shape,
+    # whose phonons live ~1200 steps); a flight that tallies its passed boundaries one step late
+    # or into the next sensor moves 9-180 %.  Both planted mistakes (an oracle test hook) must
+    # fail the same bound.
+    def moved(x):
+        return np.abs(x.counts - b.counts).sum() / np.abs(b.counts).sum()
+
+    assert moved(a) <= 1e-2, moved(a)
+    for fault in (1, 2):
+        bad = oracle.Model(fix["model"])
+        bad.plant_flight_fault(fault)
+        assert moved(bad.run(0, nthreads=8)) > 5e-2, fault
 
 
-STATS_ONLY = ["si_ge", "phasor", "needles"]
+STATS_ONLY = ["si_ge", "phasor", "needles", "kinked_15", "kinked_90"]
+# Fixtures judged as a family of entries (tests/family.py) instead of entry by entry, with the
+# reason: on seeds 0-15 the silicon|germanium bar has one x-flux at z = 3.23 while its other 29
+# entries are at most 2.34, and on seeds 16-31 and 32-47 all 30 entries are at most 2.1 -- a
+# Student-t tail, not a bias.
+FAMILY_RULE = {"si_ge"}
 
 
 @pytest.mark.parametrize("name", STATS_ONLY)
@@ -338,8 +351,10 @@ def test_oracle_statistics_match_reference_on_interfaces_and_phasors(name):
     """Paths the three workloads above do not take: transitions between two materials (a
     phonon above the neighbour's maximum frequency of its branch scatters back diffusely,
     surface.cpp:71-109), phasor mode (fixed speed and branch, no scattering,
-    phononBuilder.cpp:42-49), and acute specular needles where stuck phonons are relocated
-    (modelSimulator.cpp:215-218).  16 oracle seeds against 16 reference runs, per sensor."""
+    phononBuilder.cpp:42-49), acute specular needles where stuck phonons are relocated
+    (modelSimulator.cpp:215-218), and kinked structures of the batched-geometry family whose
+    edges lie at arbitrary slopes (geometry.cpp:104-138).  16 oracle seeds against 16 reference
+    runs, every sensor's entries within 3 standard errors (si_ge as a family, see FAMILY_RULE)."""
     with open(os.path.join(GOLDEN, f"ref_{name}.json")) as f:
         fix = json.load(f)
     ours = []
@@ -349,5 +364,8 @@ def test_oracle_statistics_match_reference_on_interfaces_and_phasors(name):
         out, _, _ = m.interpret(r.counts, r.flux)
         ours.append({"t_steady": out[:, 0], "x_flux": out[:, 2], "y_flux": out[:, 4]})
     zs = [_three_sigma(fix, ours, key) for key in ["t_steady", "x_flux", "y_flux"]]
-    ok, detail = family_ok(np.concatenate(zs), 2 * (fix["runs"] - 1))
-    assert ok, (detail, [np.round(z, 2) for z in zs])
+    if name in FAMILY_RULE:
+        ok, detail = family_ok(np.concatenate(zs), 2 * (fix["runs"] - 1))
+        assert ok, (detail, [np.round(z, 2) for z in zs])
+    else:
+        assert max(float(z.max()) for z in zs) <= 3.0, [np.round(z, 2) for z in zs]
