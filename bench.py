@@ -9,20 +9,24 @@ specularity 0.3, T_init 300 K), full (non-deviational) mode, 1e7 phonons, FP64 t
   value      phonon-steps per second of the timed runs (a phonon-step is one live phonon carried
              across one measurement interval: the reference's ++step at
              psim/src/modelSimulator.cpp:182-183), with the model already resident in HBM.
-  roofline   the transport kernel against HBM: 120 bytes per phonon-step (the phonon record
-             read and written once per step in a step-synchronous design) over its HIP-event
-             time on the engine stream.
+  roofline   the transport kernel against the step-synchronous HBM model of this path: 120 bytes
+             per phonon-step (the phonon record read and written once per step) over its
+             HIP-event time on the engine stream.  What the counters show limits it (`limiter`,
+             VALU issue, and the measured HBM bytes per phonon-step) is copied from profiles/,
+             labelled with the engine commit it was measured on and whether that is this tree's.
   cpu_baseline  the reference psim (oracle/_ref/psim_ref, built from the reference sources) on
-             a bounded sample of the same workload (1e5 phonons): one process per host core (up
-             to 16) started together, their aggregate rate (its std::execution::par loop is
-             serial in this image), with one process alone beside it; phonon-steps are counted
-             by the CPU oracle on the same model.
+             a bounded sample of the same workload (1e5 phonons): one process per CPU of the
+             job's share (the affinity set, at most 16) started together, their aggregate rate
+             (its std::execution::par loop is serial in this image), with one process alone
+             beside it and the host's CPU model, CPU count, affinity and cgroup quota;
+             phonon-steps are counted by the CPU oracle on the same model.
   time_to_steady_state  on the 1000x100 nm bar (10 sensors, walls 310/290 K, deviational, every
              step recorded, 1e7 phonons, 2000 steps of 5 ps): the step after which the RMS over
              sensors of the change in 100-step window-mean temperature stays below 0.05 K for 3
              windows (psim_amd.analysis), then the same simulation cut at that step (phonon budget
-             scaled so the energy per phonon is unchanged) timed on the GPU, and its phonon-steps
-             at the reference build's rate on this host (sampled on the same bar at 1e5 phonons).
+             scaled so the energy per phonon is unchanged) timed on the GPU.  The reference's time
+             for it is extrapolated, not run: those phonon-steps at the reference's measured rate
+             on the same bar at 1e5 phonons (one process, and all processes of the baseline).
 
 With --workload film it measures the transient case instead: the 1500x150 nm film (30 sensors,
 baseline 300 K, a 20 K burst for 100 ps into sensors 0-2), 1e8 phonons, 20000 steps of 1 ps, every
@@ -62,6 +66,58 @@ sys.path.insert(0, ROOT)
 
 BYTES_PER_PHONON_STEP = 120.0
 HBM_PEAK_GBPS = 8000.0
+ENGINE_SOURCES = ("psim_amd/csrc/engine.hip", "psim_amd/csrc/engine_device.hpp", "psim_amd/csrc/psim_rng.h",
+                  "psim_amd/csrc/psim_math.h")
+
+
+def engine_sha256() -> str:
+    """Hash of the transport engine's sources (the code a counter profile describes)."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for rel in ENGINE_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def profiled_counters():
+    """The transport kernel's counter figures on the L-shape (profiles/), with `current` telling
+    whether they were measured on the engine sources of this tree."""
+    path = os.path.join(ROOT, "profiles", "transport_kernel_lshape_1e7_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        c = json.load(f)
+    c["current"] = c.get("engine_sha256") == engine_sha256()
+    return c
+
+
+def host_cpus() -> dict:
+    """What this host gives the CPU baseline: CPU model, the CPUs the OS reports and the ones this
+    process may run on, the cgroup's CPU quota, and the processes the baseline starts (one per
+    CPU of the job's share: the affinity set, at most 16, the CPU share a one-GPU job is given on
+    the GPU hosts this was measured on)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            quota = None if q == "max" else float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    procs = max(1, min(16, affinity, int(quota) if quota else affinity))
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": affinity, "cgroup_cpu_quota": quota,
+            "processes": procs}
 
 
 def reference_rate_all_cores(cfg: dict, procs: int):
@@ -98,7 +154,7 @@ def reference_rate(cfg: dict) -> tuple:
     import oracle
     from oracle import refmodel
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cpus()["processes"]
     counted = oracle.Model(cfg).run(0, nthreads=threads).counters["phonon_steps"]
     ref_bin = os.path.join(ROOT, "oracle", "_ref", "psim_ref")
     if os.path.exists(ref_bin):
@@ -135,22 +191,22 @@ def workload(name: str, phonons: int) -> tuple:
 
 
 def cpu_baseline(name: str, sample_phonons: int) -> dict:
-    """The reference on a sample of the benchmark workload: one process per host core (up to 16)
-    started together, and one process alone."""
+    """The reference on a sample of the benchmark workload: one process per CPU of the job's share
+    started together (host_cpus), and one process alone."""
     cfg, desc = workload(name, sample_phonons)
+    host = host_cpus()
     rate, wall, counted, kind = reference_rate(cfg)
     sample = (f"{desc['workload']}, {sample_phonons} phonons x {desc['measurement_steps']} steps of "
               f"{desc['dt_ps']} ps; {counted} phonon-steps per run counted by the oracle on the same model")
     single = {"value": rate, "cores": 1, "wall_s": wall}
-    procs = max(1, min(16, os.cpu_count() or 1))
-    multi = reference_rate_all_cores(cfg, procs) if kind == "reference" else None
+    multi = reference_rate_all_cores(cfg, host["processes"]) if kind == "reference" else None
     if multi is None:
         return {"value": rate, "unit": "phonon-steps/s", "cores": 1, "kind": kind,
-                "sample": sample + ", one process", "wall_s": wall}
+                "sample": sample + ", one process", "wall_s": wall, "host": host}
     m_rate, m_wall, n = multi
     return {"value": m_rate, "unit": "phonon-steps/s", "cores": n, "kind": kind,
-            "sample": sample + f", {n} independent processes started together (one per core)", "wall_s": m_wall,
-            "one_process": single}
+            "sample": sample + f", {n} independent processes started together (one per CPU of the job's share)",
+            "wall_s": m_wall, "one_process": single, "scaling_over_one_process": m_rate / rate, "host": host}
 
 
 def steady_state(engine, phonons: int, cpu: bool) -> dict:
@@ -178,9 +234,19 @@ def steady_state(engine, phonons: int, cpu: bool) -> dict:
     ps = m.results(0).counters["phonon_steps"]
     out.update({"sim_ns": sim_ns * frac, "gpu_wall_s": gpu_s, "phonon_steps": ps})
     if cpu:
-        rate, _, _, kind = reference_rate(configs.linear_bar(num_phonons=100_000, sim_type=1, step_interval=100).to_dict())
-        out.update({"cpu_wall_s": ps / rate, "cpu_rate": rate, "cpu_kind": kind, "cpu_cores": 1,
-                    "cpu_sample": "the same bar at 1e5 phonons, one process", "speedup": ps / rate / gpu_s})
+        # Not a run: the reference's time for these phonon-steps at the rate it reaches on the same
+        # bar at 1e5 phonons (one process, and the job's share of CPUs as independent processes).
+        bar = configs.linear_bar(num_phonons=100_000, sim_type=1, step_interval=100).to_dict()
+        rate, _, _, kind = reference_rate(bar)
+        out["cpu_extrapolated"] = {
+            "method": "phonon-steps of the cut GPU run divided by the reference's measured rate on the same bar "
+                      "at 1e5 phonons; the reference run itself is not timed (about 3 core-minutes)",
+            "kind": kind, "one_process": {"rate": rate, "wall_s": ps / rate, "speedup": ps / rate / gpu_s}}
+        multi = reference_rate_all_cores(bar, host_cpus()["processes"]) if kind == "reference" else None
+        if multi is not None:
+            m_rate, _, n = multi
+            out["cpu_extrapolated"]["all_processes"] = {"processes": n, "rate": m_rate, "wall_s": ps / m_rate,
+                                                       "speedup": ps / m_rate / gpu_s}
     return out
 
 
@@ -424,7 +490,11 @@ def main() -> int:
         "method": "phonons fly across the measurement boundaries; each boundary passed is counted as a phonon-step "
                   "and, when recorded, tallied with the phonon's sensor and velocity of that moment",
         "roofline": {
+            # the step-synchronous HBM model of this path; what the counters show limits the
+            # kernel (VALU issue) is `limiter` below
             "bound": "hbm",
+            "model": "120 algorithmic bytes per phonon-step: one 60-byte SoA phonon record read and written "
+                     "per measurement step",
             "kernel": "transport_kernel",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,
@@ -442,14 +512,22 @@ def main() -> int:
     adds = 3 * counters["tallies"]
     line["tally_atomics"] = {"adds_per_run": adds,
                              "adds_per_s_of_transport_time": adds / (kernel_ms / 1e3 / args.steps)}
-    # HBM traffic of the transport kernel per launch, measured by rocprofv3 --pmc FETCH_SIZE and
-    # WRITE_SIZE (scaled by the copy-kernel calibration) on this workload; see profiles/.
-    traffic_file = os.path.join(ROOT, "profiles", "transport_kernel_lshape_1e7_traffic.json")
-    if not film and args.phonons == 10_000_000 and os.path.exists(traffic_file):
-        with open(traffic_file) as f:
-            tr = json.load(f)
-        line["roofline"]["traffic"] = tr["bytes_per_dispatch"]
-        line["roofline"]["traffic_bytes_per_phonon_step"] = tr["bytes_per_phonon_step"]
+    # What the counters say of this kernel on this workload (rocprofv3 --pmc, profiles/): HBM bytes
+    # per phonon-step (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and VALU
+    # issue.  Counters cannot be read inside this process, so these figures are copied from the
+    # profile, labelled with the engine they were measured on and whether it is this one.
+    if not film and args.phonons == 10_000_000:
+        copied = profiled_counters()
+        if copied is not None:
+            per_launch = steps_total / max(1, launches)
+            line["roofline"]["traffic"] = copied["traffic"]["bytes_per_phonon_step"] * per_launch
+            line["roofline"]["traffic_bytes_per_phonon_step"] = copied["traffic"]["bytes_per_phonon_step"]
+            line["roofline"]["limiter"] = copied["limiter"]
+            line["roofline"]["issue"] = copied["issue"]
+            line["roofline"]["counters_copied_from"] = {
+                "file": "profiles/transport_kernel_lshape_1e7_counters.json",
+                "engine_commit": copied["engine_commit"], "engine_is_current": copied["current"],
+                "traffic": "bytes per phonon-step from the profile x this run's phonon-steps per launch"}
     # the boundary's own call end to end: host tables and budget, upload, run, tally copy-back and
     # interpretation (psim_model_run); reported beside `value`, which has the model resident in HBM
     if not film:

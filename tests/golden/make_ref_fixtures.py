@@ -13,6 +13,11 @@ own sources plus oracle/ref_driver.cpp) and runs it on these workloads:
   phasor        the bar in phasor mode (statistics only)
   needles       the bar with six specular needles (configs.needle_bar): acute corners where
                 stuck phonons are relocated (statistics only)
+  linear_b      the linear_ss bar at 6e6 phonons, above the reference's PHONON_CUTOFF (5e6), so
+                its runs take the builder path (statistics only)
+  lshape_full   the L-shape at the benchmark's 1e7 phonons (statistics only; ~25 core-minutes
+                per run)
+  kinked_15, kinked_90  kinked structures of the batched-geometry family (statistics only)
 
 For each workload it stores
   ref_<name>.json  the model, and over N independent reference runs (the
@@ -63,6 +68,12 @@ def workloads():
         # the linear_ss bar above the reference's PHONON_CUTOFF of 5e6 (modelSimulator.cpp:18), so
         # that its runs take runUsingBuilders (:245-253) instead of runPhononByPhonon
         "linear_b": (configs.linear_ss(num_phonons=6_000_000), None),
+        # the benchmark workload at its own size (1e7 phonons, builder path): in full mode the
+        # interpreter's bisection is bounded by [min - 10, max + 10] K (sensorInterpreter.cpp:87-95),
+        # and the sensors next to the hot wall reach that bound, so their mean temperature depends
+        # on the per-step noise and with it on the phonon count; only a fixture of this size
+        # pins them
+        "lshape_full": (configs.lshape_full(num_phonons=10_000_000), None),
         # kinked structures of the batched-geometry family: arms at 15 and 90 degrees
         "kinked_15": (configs.kinked_structure(15.0), None),
         "kinked_90": (configs.kinked_structure(90.0), None),
