@@ -159,19 +159,13 @@ __global__ void __launch_bounds__(BLOCK) emit_kernel(Model m, Pool pool, int64_t
             key = (int32_t)step;
         }
         birth_key[id] = key;
-        pool.px[id] = p.px;
-        pool.py[id] = p.py;
-        pool.dx[id] = p.dx;
-        pool.dy[id] = p.dy;
-        pool.freq[id] = p.freq;
-        pool.vel[id] = p.vel;
-        pool.age[id] = birth;
-        pool.tts[id] = 0.;
-        pool.cell[id] = cell;
-        pool.sensor[id] = cl.sensor;
-        pool.sid[id] = sid;
-        pool.meta[id] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
-                        | ((uint32_t)src.stream << META_STREAM_SHIFT);
+        pool.pos[id] = make_double2(p.px, p.py);
+        pool.dir[id] = make_double2(p.dx, p.dy);
+        pool.fv[id] = make_double2(p.freq, p.vel);
+        pool.aux[id] = PoolAux{ birth, cell, cl.sensor };
+        pool.ms[id] = make_uint2((uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
+                                     | ((uint32_t)src.stream << META_STREAM_SHIFT),
+            sid);
     }
     // birth-step histogram; most waves of a cell-origin source share one key (step 0)
     const int lane = threadIdx.x % WAVE;
@@ -234,25 +228,30 @@ __global__ void __launch_bounds__(BLOCK) birth_scatter_kernel(const int32_t* __r
 // State of the phonons that are alive between two launch windows, densely packed in the order
 // the previous launch paused them, with the phonon id (the key of its Philox stream).  A launch
 // reads its started phonons from one buffer in work-queue order and appends its survivors to the
-// other, so the phonons that neighbouring lanes and waves fetch or park share cache lines.
+// other, so the phonons that neighbouring lanes and waves fetch or park share cache lines.  Seven
+// arrays of 16-byte records (112 B per phonon): a lane moves a phonon in seven vector loads or
+// stores, and a launch holds seven pointers per buffer.
+struct __attribute__((aligned(16))) LiveAux {
+    double rate2;  // third relaxation rate of the last scattering-time draw
+    int32_t cell;  // -1: a hole (a reserved slot nobody filled)
+    int32_t sensor;  // sensor of the cell
+};
+
+struct __attribute__((aligned(16))) LiveTag {
+    uint32_t meta;
+    uint32_t id;
+    uint32_t sid;  // id within the phonon's random stream
+    int32_t step;  // measurement step (past the window's end after a long flight)
+};
+
 struct Live {
-    double* __restrict__ px;
-    double* __restrict__ py;
-    double* __restrict__ dx;
-    double* __restrict__ dy;
-    double* __restrict__ freq;
-    double* __restrict__ vel;
-    double* __restrict__ age;
-    double* __restrict__ tts;
-    double* __restrict__ rate0;  // relaxation rates of the last scattering-time draw
-    double* __restrict__ rate1;
-    double* __restrict__ rate2;
-    int32_t* __restrict__ cell;
-    int32_t* __restrict__ sensor;  // sensor of the cell
-    uint32_t* __restrict__ meta;
-    uint32_t* __restrict__ id;
-    uint32_t* __restrict__ sid;  // id within the phonon's random stream
-    int32_t* __restrict__ step;  // measurement step (past the window's end after a long flight)
+    double2* __restrict__ pos;  // px, py
+    double2* __restrict__ dir;  // dx, dy
+    double2* __restrict__ fv;  // freq, vel
+    double2* __restrict__ at;  // age, time to the next scatter
+    double2* __restrict__ r01;  // first two relaxation rates of the last scattering-time draw
+    LiveAux* __restrict__ aux;
+    LiveTag* __restrict__ tag;
 };
 
 // Work of one transport launch: the started phonons (Live buffer `in`, *n_in entries) followed by
@@ -503,29 +502,52 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                 // started phonons from the dense buffer, phonons born in the window from the pool
                 const bool started = item < n_live;
                 const Live& L = lists.in;
-                slot = started ? L.id[item] : lists.born_order[b0 + (item - n_live)];
-                const int64_t j = started ? item : (int64_t)slot;
-                const uint32_t meta = (started ? L.meta : pool.meta)[j];
-                const uint32_t sid = MULTI ? (started ? L.sid : pool.sid)[j] : slot;
-                p.cell = (started ? L.cell : pool.cell)[j];
-                cur_sensor = (started ? L.sensor : pool.sensor)[j];
-                p.px = (started ? L.px : pool.px)[j];
-                p.py = (started ? L.py : pool.py)[j];
-                p.dx = (started ? L.dx : pool.dx)[j];
-                p.dy = (started ? L.dy : pool.dy)[j];
-                p.freq = (started ? L.freq : pool.freq)[j];
-                p.vel = (started ? L.vel : pool.vel)[j];
-                age = (started ? L.age : pool.age)[j];
-                tts = (started ? L.tts : pool.tts)[j];
-                if (started) {  // rates of the last scattering-time draw (a new phonon has none yet)
-                    relax[0] = L.rate0[item];
-                    relax[1] = L.rate1[item];
-                    relax[2] = L.rate2[item];
+                uint32_t meta, sid;
+                double2 pos, dir, fv;
+                if (started) {
+                    const LiveTag tg = L.tag[item];
+                    slot = tg.id;
+                    meta = tg.meta;
+                    sid = tg.sid;
+                    p.step = tg.step;
+                    pos = L.pos[item];
+                    dir = L.dir[item];
+                    fv = L.fv[item];
+                    const double2 at = L.at[item];
+                    age = at.x;
+                    tts = at.y;
+                    const double2 r = L.r01[item];  // rates of the last scattering-time draw
+                    const LiveAux ax = L.aux[item];
+                    relax[0] = r.x;
+                    relax[1] = r.y;
+                    relax[2] = ax.rate2;
+                    p.cell = ax.cell;
+                    cur_sensor = ax.sensor;
+                } else {  // a new phonon has no scattering time and no rates yet
+                    slot = lists.born_order[b0 + (item - n_live)];
+                    const uint2 ms = pool.ms[slot];
+                    meta = ms.x;
+                    sid = ms.y;
+                    pos = pool.pos[slot];
+                    dir = pool.dir[slot];
+                    fv = pool.fv[slot];
+                    const PoolAux ax = pool.aux[slot];
+                    age = ax.age;
+                    tts = 0.;
+                    p.cell = ax.cell;
+                    cur_sensor = ax.sensor;
+                    p.step = (int32_t)(age / m.step_time);
                 }
+                if (!MULTI) { sid = slot; }
+                p.px = pos.x;
+                p.py = pos.y;
+                p.dx = dir.x;
+                p.dy = dir.y;
+                p.freq = fv.x;
+                p.vel = fv.y;
                 p.bin = (int)(meta & META_BIN_MASK);
                 p.pol = (meta & META_TA) ? POL_TA : POL_LA;
                 p.sign = (meta & META_NEG) ? -1 : 1;
-                p.step = started ? L.step[item] : (int32_t)(age / m.step_time);
                 ttm = 0.;
                 events = 0;
                 // the stream keeps (stream index, id within it) in its counter words c3 and c2
@@ -655,24 +677,15 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
             }
             if (survive) {
                 const Live& O = lists.out;
-                O.px[k] = p.px;
-                O.py[k] = p.py;
-                O.dx[k] = p.dx;
-                O.dy[k] = p.dy;
-                O.freq[k] = p.freq;
-                O.vel[k] = p.vel;
-                O.age[k] = age;
-                O.tts[k] = tts;
-                O.rate0[k] = relax[0];
-                O.rate1[k] = relax[1];
-                O.rate2[k] = relax[2];
-                O.cell[k] = p.cell;
-                O.sensor[k] = cur_sensor;
-                O.meta[k] = (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
-                            | META_STARTED | (MULTI ? (st.c3 >> 8) << META_STREAM_SHIFT : 0u);
-                O.id[k] = slot;
-                O.step[k] = p.step;
-                if (MULTI) { O.sid[k] = st.c2; }
+                O.pos[k] = make_double2(p.px, p.py);
+                O.dir[k] = make_double2(p.dx, p.dy);
+                O.fv[k] = make_double2(p.freq, p.vel);
+                O.at[k] = make_double2(age, tts);
+                O.r01[k] = make_double2(relax[0], relax[1]);
+                O.aux[k] = LiveAux{ relax[2], p.cell, cur_sensor };
+                O.tag[k] = LiveTag{ (uint32_t)p.bin | (p.pol == POL_TA ? META_TA : 0u) | (p.sign < 0 ? META_NEG : 0u)
+                                        | META_STARTED | (MULTI ? (st.c3 >> 8) << META_STREAM_SHIFT : 0u),
+                    slot, MULTI ? st.c2 : slot, p.step };
                 state = S_FETCH;
             }
         }
@@ -684,7 +697,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     }
 
     // Reserved survivor slots the wave did not fill become holes.
-    for (uint32_t k = out_next + (uint32_t)lane; k < out_end; k += WAVE) { lists.out.cell[k] = -1; }
+    for (uint32_t k = out_next + (uint32_t)lane; k < out_end; k += WAVE) { lists.out.aux[k].cell = -1; }
 
     // Counters: wave sums, then one global atomic per wave and counter.
     const uint32_t v[8] = { c.steps, c.scatters, c.impacts, c.relocations, c.absorbed, c.expired, c.tallies,
@@ -722,7 +735,7 @@ __device__ __forceinline__ void sort_range(uint32_t n, uint32_t& lo, uint32_t& h
     hi = (uint32_t)std::min<uint64_t>(n, (uint64_t)lo + per);
 }
 
-__global__ void __launch_bounds__(SORT_BLOCK) cell_hist_kernel(const int32_t* __restrict__ cell,
+__global__ void __launch_bounds__(SORT_BLOCK) cell_hist_kernel(const LiveAux* __restrict__ aux,
     const uint32_t* __restrict__ n_ptr, int shift, int n_keys, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[SORT_KEYS];
     for (int k = threadIdx.x; k < n_keys; k += SORT_BLOCK) { h[k] = 0; }
@@ -730,7 +743,7 @@ __global__ void __launch_bounds__(SORT_BLOCK) cell_hist_kernel(const int32_t* __
     uint32_t lo, hi;
     sort_range(*n_ptr, lo, hi);
     for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_BLOCK) {
-        const int32_t cl = cell[i];
+        const int32_t cl = aux[i].cell;
         if (cl >= 0) { atomicAdd(&h[(uint32_t)cl >> shift], 1u); }
     }
     __syncthreads();
@@ -770,7 +783,7 @@ __global__ void __launch_bounds__(1024) cell_scan_kernel(uint32_t* __restrict__ 
 }
 
 __global__ void __launch_bounds__(SORT_BLOCK) cell_scatter_kernel(Live src, Live dst,
-    const uint32_t* __restrict__ n_ptr, int shift, int n_keys, uint32_t* __restrict__ cursor, int copy_sid) {
+    const uint32_t* __restrict__ n_ptr, int shift, int n_keys, uint32_t* __restrict__ cursor) {
     __shared__ uint32_t cnt[SORT_KEYS];
     __shared__ uint32_t base[SORT_KEYS];
     for (int k = threadIdx.x; k < n_keys; k += SORT_BLOCK) { cnt[k] = 0; }
@@ -778,7 +791,7 @@ __global__ void __launch_bounds__(SORT_BLOCK) cell_scatter_kernel(Live src, Live
     uint32_t lo, hi;
     sort_range(*n_ptr, lo, hi);
     for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_BLOCK) {
-        const int32_t cl = src.cell[i];
+        const int32_t cl = src.aux[i].cell;
         if (cl >= 0) { atomicAdd(&cnt[(uint32_t)cl >> shift], 1u); }
     }
     __syncthreads();
@@ -788,27 +801,17 @@ __global__ void __launch_bounds__(SORT_BLOCK) cell_scatter_kernel(Live src, Live
     }
     __syncthreads();
     for (uint32_t i = lo + threadIdx.x; i < hi; i += SORT_BLOCK) {
-        const int32_t cl = src.cell[i];
-        if (cl < 0) { continue; }  // hole
-        const uint32_t key = (uint32_t)cl >> shift;
+        const LiveAux ax = src.aux[i];
+        if (ax.cell < 0) { continue; }  // hole
+        const uint32_t key = (uint32_t)ax.cell >> shift;
         const uint32_t k = base[key] + atomicAdd(&cnt[key], 1u);
-        dst.px[k] = src.px[i];
-        dst.py[k] = src.py[i];
-        dst.dx[k] = src.dx[i];
-        dst.dy[k] = src.dy[i];
-        dst.freq[k] = src.freq[i];
-        dst.vel[k] = src.vel[i];
-        dst.age[k] = src.age[i];
-        dst.tts[k] = src.tts[i];
-        dst.rate0[k] = src.rate0[i];
-        dst.rate1[k] = src.rate1[i];
-        dst.rate2[k] = src.rate2[i];
-        dst.cell[k] = cl;
-        dst.sensor[k] = src.sensor[i];
-        dst.meta[k] = src.meta[i];
-        dst.id[k] = src.id[i];
-        dst.step[k] = src.step[i];
-        if (copy_sid) { dst.sid[k] = src.sid[i]; }
+        dst.pos[k] = src.pos[i];
+        dst.dir[k] = src.dir[i];
+        dst.fv[k] = src.fv[i];
+        dst.at[k] = src.at[i];
+        dst.r01[k] = src.r01[i];
+        dst.aux[k] = ax;
+        dst.tag[k] = src.tag[i];
     }
 }
 
@@ -939,8 +942,7 @@ void free_model(psim_gpu_ctx* c) {
 }
 
 void free_pool(psim_gpu_ctx* c) {
-    void* ps[] = { c->pool.px, c->pool.py, c->pool.dx, c->pool.dy, c->pool.freq, c->pool.vel, c->pool.age, c->pool.tts,
-        c->pool.cell, c->pool.sensor, c->pool.meta, c->pool.sid };
+    void* ps[] = { c->pool.pos, c->pool.dir, c->pool.fv, c->pool.aux, c->pool.ms };
     for (void* p : ps) {
         if (p) { (void)hipFree(p); }
     }
@@ -962,8 +964,7 @@ void free_lists(psim_gpu_ctx* c) {
         if (p) { (void)hipFree(p); }
     }
     for (Live& L : c->live) {
-        void* ls[] = { L.px, L.py, L.dx, L.dy, L.freq, L.vel, L.age, L.tts, L.rate0, L.rate1, L.rate2, L.cell, L.sensor,
-            L.meta, L.id, L.sid, L.step };
+        void* ls[] = { L.pos, L.dir, L.fv, L.at, L.r01, L.aux, L.tag };
         for (void* p : ls) {
             if (p) { (void)hipFree(p); }
         }
@@ -1074,20 +1075,12 @@ int prepare_run(psim_gpu_ctx* c, RunPlan& plan) {
     if (c->pool_cap < n) {
         free_pool(c);
         const size_t cap = (size_t)std::max<int64_t>(n, 1);
-        c->pool.px = dalloc<double>(cap);
-        c->pool.py = dalloc<double>(cap);
-        c->pool.dx = dalloc<double>(cap);
-        c->pool.dy = dalloc<double>(cap);
-        c->pool.freq = dalloc<double>(cap);
-        c->pool.vel = dalloc<double>(cap);
-        c->pool.age = dalloc<double>(cap);
-        c->pool.tts = dalloc<double>(cap);
-        c->pool.cell = dalloc<int32_t>(cap);
-        c->pool.sensor = dalloc<int32_t>(cap);
-        c->pool.meta = dalloc<uint32_t>(cap);
-        c->pool.sid = dalloc<uint32_t>(cap);
-        if (!c->pool.px || !c->pool.py || !c->pool.dx || !c->pool.dy || !c->pool.freq || !c->pool.vel || !c->pool.age
-            || !c->pool.tts || !c->pool.cell || !c->pool.sensor || !c->pool.meta || !c->pool.sid) {
+        c->pool.pos = dalloc<double2>(cap);
+        c->pool.dir = dalloc<double2>(cap);
+        c->pool.fv = dalloc<double2>(cap);
+        c->pool.aux = dalloc<PoolAux>(cap);
+        c->pool.ms = dalloc<uint2>(cap);
+        if (!c->pool.pos || !c->pool.dir || !c->pool.fv || !c->pool.aux || !c->pool.ms) {
             free_pool(c);
             return fail(c, PSIM_E_DEVICE, "hipMalloc failed for the phonon pool");
         }
@@ -1126,25 +1119,14 @@ int prepare_run(psim_gpu_ctx* c, RunPlan& plan) {
         c->d_born_order = dalloc<uint32_t>(cap);
         bool ok = c->d_birth_key && c->d_born_order;
         for (Live& L : c->live) {
-            L.px = dalloc<double>(cap);
-            L.py = dalloc<double>(cap);
-            L.dx = dalloc<double>(cap);
-            L.dy = dalloc<double>(cap);
-            L.freq = dalloc<double>(cap);
-            L.vel = dalloc<double>(cap);
-            L.age = dalloc<double>(cap);
-            L.tts = dalloc<double>(cap);
-            L.rate0 = dalloc<double>(cap);
-            L.rate1 = dalloc<double>(cap);
-            L.rate2 = dalloc<double>(cap);
-            L.cell = dalloc<int32_t>(cap);
-            L.sensor = dalloc<int32_t>(cap);
-            L.meta = dalloc<uint32_t>(cap);
-            L.id = dalloc<uint32_t>(cap);
-            L.sid = dalloc<uint32_t>(cap);
-            L.step = dalloc<int32_t>(cap);
-            ok = ok && L.px && L.py && L.dx && L.dy && L.freq && L.vel && L.age && L.tts && L.rate0 && L.rate1 && L.rate2
-                 && L.cell && L.sensor && L.meta && L.id && L.sid && L.step;
+            L.pos = dalloc<double2>(cap);
+            L.dir = dalloc<double2>(cap);
+            L.fv = dalloc<double2>(cap);
+            L.at = dalloc<double2>(cap);
+            L.r01 = dalloc<double2>(cap);
+            L.aux = dalloc<LiveAux>(cap);
+            L.tag = dalloc<LiveTag>(cap);
+            ok = ok && L.pos && L.dir && L.fv && L.at && L.r01 && L.aux && L.tag;
         }
         if (!ok) {
             free_lists(c);
@@ -1298,14 +1280,14 @@ int enqueue_run(psim_gpu_ctx* c, const RunPlan& plan, bool eager) {
             const uint32_t* n_ptr = c->d_n_live + w + 1;
             const int sgrid = std::max(1, c->n_cu * 2);
             if ((rc = zero_fill(c, c->d_sort_keys, sizeof(uint32_t) * (size_t)n_keys)) != 0) { return rc; }
-            hipLaunchKernelGGL(cell_hist_kernel, dim3(sgrid), dim3(SORT_BLOCK), 0, c->stream, c->live[1].cell, n_ptr,
+            hipLaunchKernelGGL(cell_hist_kernel, dim3(sgrid), dim3(SORT_BLOCK), 0, c->stream, c->live[1].aux, n_ptr,
                 shift, n_keys, c->d_sort_keys);
             HIP_CHECK(c, hipGetLastError());
             hipLaunchKernelGGL(cell_scan_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_sort_keys, n_keys,
                 c->d_sort_keys + SORT_KEYS);
             HIP_CHECK(c, hipGetLastError());
             hipLaunchKernelGGL(cell_scatter_kernel, dim3(sgrid), dim3(SORT_BLOCK), 0, c->stream, c->live[1], c->live[0],
-                n_ptr, shift, n_keys, c->d_sort_keys, c->multi_stream ? 1 : 0);
+                n_ptr, shift, n_keys, c->d_sort_keys);
             HIP_CHECK(c, hipGetLastError());
         }
         if (!eager) { continue; }

@@ -76,19 +76,22 @@ struct Model {
     double t_eq;
 };
 
+// Birth time, cell (-1: never starts) and the cell's sensor of a pooled phonon.
+struct __attribute__((aligned(16))) PoolAux {
+    double age;
+    int32_t cell;
+    int32_t sensor;
+};
+
+// The emission pool, indexed by phonon id: five arrays of 8- and 16-byte records, so that a lane
+// moves a phonon in five vector loads and the kernels keep five pointers instead of twelve.  A
+// newborn phonon has no scattering time yet (0).
 struct Pool {
-    double* __restrict__ px;
-    double* __restrict__ py;
-    double* __restrict__ dx;
-    double* __restrict__ dy;
-    double* __restrict__ freq;
-    double* __restrict__ vel;
-    double* __restrict__ age;
-    double* __restrict__ tts;
-    int32_t* __restrict__ cell;
-    int32_t* __restrict__ sensor;  // sensor of the cell
-    uint32_t* __restrict__ meta;
-    uint32_t* __restrict__ sid;  // id within the phonon's random stream (id - source id_base)
+    double2* __restrict__ pos;  // px, py
+    double2* __restrict__ dir;  // dx, dy
+    double2* __restrict__ fv;  // freq, vel
+    PoolAux* __restrict__ aux;
+    uint2* __restrict__ ms;  // meta, id within the phonon's random stream (id - source id_base)
 };
 
 struct Ph {
