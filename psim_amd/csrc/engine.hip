@@ -286,7 +286,7 @@ constexpr int TALLY_NONE = 0, TALLY_LDS = 1, TALLY_GLOBAL = 2;
 #endif
 
 // Lane states of the scheduled event loop.
-enum : int { S_FETCH = 0, S_TTS = 1, S_NI = 2, S_COLL = 3, S_SCAT = 4, S_EXIT = 5, S_PAUSED = 6, S_DEAD = 7 };
+enum : int { S_FETCH = 0, S_TTS = 1, S_NI = 2, S_COLL = 3, S_SCAT = 4, S_END = 5, S_EXIT = 6, S_PAUSED = 7, S_DEAD = 8 };
 
 // Advances the listed phonons across measurement steps [k0, k1).  Each lane carries one phonon
 // through ModelSimulator::simulatePhonon (psim/src/modelSimulator.cpp:139-198), pausing right
@@ -431,6 +431,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     // with no draw made (so a resumed phonon continues exactly), at the latest at boundary
     // `stop`, where every window before the recorded steps ends.
     bool relocated = false;  // the flight ended frozen after a relocation (same sensor, same scattering time)
+    double end_arg = 0.;  // S_END: handleImpacts' result, applied by end_drift
     auto end_drift = [&](double result) {
         if (cur_sensor != sensor0) { drift_time = result; }
         const double factor = p.vel * (drift_time - result);  // Phonon::drift
@@ -568,7 +569,8 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
             if (find_impact(m, p, drift_time - drifted, t_hit, c)) {
                 state = S_COLL;
             } else {
-                end_drift(drifted);
+                end_arg = drifted;
+                state = S_END;
             }
         }
     };
@@ -602,14 +604,23 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                     const double frozen = (double)(p.step + 1) * m.sim_time / (double)m.N - age;
                     if (frozen < drift_time) { drift_time = frozen; }
                     relocated = true;
-                    end_drift(drift_time);
-                    relocated = false;
+                    end_arg = drift_time;
+                    state = S_END;
                 } else if (cur_sensor != sensor0) {
-                    end_drift(drifted);
+                    end_arg = drifted;
+                    state = S_END;
                 } else {
                     state = S_NI;
                 }
             }
+        }
+    };
+    // The rest of simulatePhonon's loop body once handleImpacts has returned (end_drift), for the
+    // lanes whose drift ended in this iteration's impact search or collision.
+    auto stage_end = [&]() {
+        if (state == S_END) {
+            end_drift(end_arg);
+            relocated = false;
         }
     };
     // Scatter (lanes in S_SCAT), then the scattering-time draw (lanes in S_SCAT or S_TTS).
@@ -638,21 +649,20 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     };
 
     for (;;) {
-        // Every iteration runs the stages in the order a phonon most often passes through them --
-        // refill, surface collision, scatter and scattering-time draw, impact search -- each for
-        // the lanes waiting in it, so that lanes that leave one stage join the next one in the
-        // same iteration: a collision that changes sensor needs a scattering-time draw, a scatter
-        // an impact search, and a wave alternating between them runs each with most of its lanes
-        // instead of one of them with the lanes that wait for it (on the 1e7 L-shape, 5.47e10
-        // phonon-steps/s against 4.78e10 with one stage per iteration, the one with most waiting
-        // lanes, and 5.18e10 with scatters before collisions).  Branches are on wave-uniform
-        // ballots; a stage with no waiting lane is skipped.
+        // Every iteration runs the stages in the order a phonon passes through them -- refill, one
+        // impact search, the collision after a hit, the end of a drift, scatter and the
+        // scattering-time draw -- each for the lanes waiting in it, so that lanes that leave one
+        // stage join the next one in the same iteration and each stage's code runs once per
+        // iteration for all of them: a wave alternating between impact searches and scatters
+        // runs each with most of its lanes instead of one of them with the lanes that wait for
+        // it.  Branches are on wave-uniform ballots; a stage with no waiting lane is skipped.
         if (!__ballot(state < S_EXIT)) { break; }  // every lane has exited
         const uint64_t m_fetch = __ballot(state == S_FETCH);
         if (m_fetch) { stage_fetch(m_fetch); }
-        if (__ballot(state == S_COLL)) { stage_coll(); }
-        if (__ballot(state == S_SCAT || state == S_TTS)) { stage_scat(); }
         if (__ballot(state == S_NI)) { stage_ni(); }
+        if (__ballot(state == S_COLL)) { stage_coll(); }
+        if (__ballot(state == S_END)) { stage_end(); }
+        if (__ballot(state == S_SCAT || state == S_TTS)) { stage_scat(); }
 
         // Phonons that paused this iteration go to the next window's buffer, in slots the wave
         // has reserved; the dead leave.

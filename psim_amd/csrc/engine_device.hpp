@@ -379,15 +379,16 @@ __device__ __forceinline__ void relax_rates(const Model& m, int sensor, const Ph
 
 // ModelSimulator::scatter, modelSimulator.cpp:124-137, with its draw `rand` taken by the caller
 // (together with the next scattering-time draw, one block).
+// The new direction of an Umklapp scatter (after the resample) and of an impurity scatter is one
+// code path, so a wave holding both runs it once.
 __device__ __forceinline__ void scatter(const Model& m, Ph& p, const double* r, double rand, psim_stream& st) {
     const double tau_inv = 0. + r[0] + r[1] + r[2];
-    if (rand <= (r[0] + r[1]) / tau_inv) {
+    const bool resampled = rand <= (r[0] + r[1]) / tau_inv;
+    if (resampled) {
         const psim_sensor& s = m.sensors[m.cells[p.cell].sensor];
         resample(m, s.material, s.scatter_table, p, st);
-        if (rand > r[0] / tau_inv) { set_rand_direction(p, st); }
-    } else if (r[2] > 0.) {
-        set_rand_direction(p, st);
     }
+    if (resampled ? rand > r[0] / tau_inv : r[2] > 0.) { set_rand_direction(p, st); }
 }
 
 }  // namespace psim_dev
