@@ -37,6 +37,16 @@ int psim_mlp_get_params(psim_mlp* mlp, float* params);
  * blocks of 64 rows); more is rejected with PSIM_E_INVALID and a message. */
 int psim_mlp_forward(psim_mlp* mlp, const float* x, int32_t rows, float* y);
 
+/* psim_mlp_forward without any device allocation: no counterpart in the reference; for callers
+ * that cannot allocate device memory (tight serving loops, harnesses that forbid hipMalloc).  The
+ * same kernels and launch shapes as psim_mlp_forward, bit-identical results, with every buffer beyond
+ * the caller's x and y carved from a 64 MiB device array that the library holds on each GPU it runs
+ * on until the process ends.  Takes at most as many rows as that array holds for this network
+ * (15616 for 32 -> 512 x 4); more is refused with PSIM_E_INVALID and a message.  Every call on one GPU
+ * uses that GPU's one array: calls that overlap on one GPU (two threads, or two networks on
+ * different streams) race. */
+int psim_mlp_forward_noalloc(psim_mlp* mlp, const float* x, int32_t rows, float* y);
+
 /* Mean squared error against t and its gradient with respect to every parameter (FP32). */
 int psim_mlp_gradients(psim_mlp* mlp, const float* x, const float* t, int32_t rows, float* grad, float* loss);
 

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -324,7 +325,22 @@ __global__ void to_bf16_kernel(const float* __restrict__ x, uint16_t* __restrict
     if (i < n) { y[i] = f32_to_bf16(x[i]); }
 }
 
+// y[r] = z[r][0] for a [rows][cols] matrix.
+__global__ void column0_kernel(const float* __restrict__ z, float* __restrict__ y, int rows, int cols) {
+    const int r = blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (r < rows) { y[r] = z[(int64_t)r * cols]; }
+}
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+
 }  // namespace
+
+// Device memory of psim_mlp_forward_noalloc (input staging, the bf16 activations, one FP32
+// pre-activation buffer and the output column), carved from this array instead of allocated.
+// Every GPU the library runs on holds its own copy from the moment the library's code is loaded
+// there until the process ends.  In host code `scratch` names a host-side stand-in, not device
+// memory: host code uses only the per-device address that hipGetSymbolAddress returns for it.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
 struct psim_mlp {
     int device = 0;
@@ -350,6 +366,8 @@ struct psim_mlp {
     float* xin = nullptr;
     float* t = nullptr;
     float* loss = nullptr;
+    // device address of this GPU's `scratch` array (psim_mlp_forward_noalloc), looked up at create
+    unsigned char* scratch = nullptr;
 };
 
 namespace {
@@ -485,6 +503,70 @@ int load_batch(psim_mlp* h, const float* x, const float* t, int rows) {
     return PSIM_OK;
 }
 
+// Device address of this GPU's scratch array, looked up once per device with that device current.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mutex;
+unsigned char* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int noalloc_scratch(psim_mlp* h) {
+    if (h->device < 0 || h->device >= NOALLOC_MAX_DEVICES) {
+        return mlp_fail(h, PSIM_E_INVALID, "device index beyond the scratch table");
+    }
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (!noalloc_ptr[h->device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            return mlp_fail(h, PSIM_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        }
+        noalloc_ptr[h->device] = static_cast<unsigned char*>(ptr);
+    }
+    h->scratch = noalloc_ptr[h->device];
+    return PSIM_OK;
+}
+
+// Layout of the no-allocation forward pass in the scratch array for `rows` rows (a multiple of
+// BATCH_ROUND): every piece starts on a 256-byte boundary, as hipMalloc'd buffers do, so the GEMM
+// takes the same vector paths as in psim_mlp_forward.
+struct NoallocLayout {
+    size_t xin, a0, act0, act1, z, y, end;
+};
+
+NoallocLayout noalloc_layout(const psim_mlp* h, int64_t rows) {
+    const auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+    const size_t wide = (size_t)std::max(h->hidden, PAD_OUT);
+    NoallocLayout l{};
+    size_t at = 0;
+    l.xin = at;
+    at = up(at + (size_t)rows * h->n_in * sizeof(float));
+    l.a0 = at;
+    at = up(at + (size_t)rows * h->n_in * sizeof(uint16_t));
+    l.act0 = at;
+    at = up(at + (size_t)rows * wide * sizeof(uint16_t));
+    l.act1 = at;
+    at = up(at + (size_t)rows * wide * sizeof(uint16_t));
+    l.z = at;
+    at = up(at + (size_t)rows * wide * sizeof(float));
+    l.y = at;
+    at = up(at + (size_t)rows * sizeof(float));
+    l.end = at;
+    return l;
+}
+
+// The most rows one no-allocation call takes for this network.
+int noalloc_max_rows(const psim_mlp* h) {
+    int64_t lo = 0, hi = MAX_ROWS / BATCH_ROUND;  // in units of BATCH_ROUND rows
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) / 2;
+        if (noalloc_layout(h, mid * BATCH_ROUND).end <= NOALLOC_SCRATCH_BYTES) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return (int)(lo * BATCH_ROUND);
+}
+
 int read_loss(psim_mlp* h, int rows, float* loss) {
     std::vector<float> terms((size_t)rows);
     MLP_CHECK(h, hipMemcpyAsync(terms.data(), h->loss, sizeof(float) * (size_t)rows, hipMemcpyDeviceToHost, h->stream));
@@ -555,6 +637,10 @@ int psim_mlp_create(psim_mlp** out, int device, int32_t n_in, int32_t hidden, in
         psim_mlp_destroy(h);
         return PSIM_E_DEVICE;
     }
+    if (noalloc_scratch(h) != PSIM_OK) {
+        psim_mlp_destroy(h);
+        return PSIM_E_DEVICE;
+    }
     *out = h;
     return PSIM_OK;
 }
@@ -619,6 +705,48 @@ int psim_mlp_forward(psim_mlp* h, const float* x, int32_t rows, float* y) {
     MLP_CHECK(h, hipMemcpyAsync(z.data(), h->z[L - 1], sizeof(float) * z.size(), hipMemcpyDeviceToHost, h->stream));
     MLP_CHECK(h, hipStreamSynchronize(h->stream));
     for (int r = 0; r < rows; ++r) { y[r] = z[(size_t)r * PAD_OUT]; }
+    return PSIM_OK;
+}
+
+int psim_mlp_forward_noalloc(psim_mlp* h, const float* x, int32_t rows, float* y) {
+    if (!h || !x || !y || rows <= 0) { return PSIM_E_INVALID; }
+    const int max_rows = noalloc_max_rows(h);
+    if (rows > max_rows) {
+        return mlp_fail(h, PSIM_E_INVALID,
+            "psim_mlp_forward_noalloc: " + std::to_string(rows) + " rows exceed the " + std::to_string(max_rows)
+                + " this network fits in the 64 MiB scratch array; split the call or use psim_mlp_forward");
+    }
+    if (!h->scratch) { return mlp_fail(h, PSIM_E_STATE, "scratch address unknown"); }
+    MLP_CHECK(h, hipSetDevice(h->device));
+    const int64_t cap = ((int64_t)rows + BATCH_ROUND - 1) / BATCH_ROUND * BATCH_ROUND;
+    const NoallocLayout lay = noalloc_layout(h, cap);
+    float* xin = reinterpret_cast<float*>(h->scratch + lay.xin);
+    uint16_t* act[3] = { reinterpret_cast<uint16_t*>(h->scratch + lay.a0),
+        reinterpret_cast<uint16_t*>(h->scratch + lay.act0), reinterpret_cast<uint16_t*>(h->scratch + lay.act1) };
+    float* z = reinterpret_cast<float*>(h->scratch + lay.z);
+    float* yd = reinterpret_cast<float*>(h->scratch + lay.y);
+    // the kernels and launch shapes of forward(), with one pre-activation buffer and two
+    // activation buffers used in turn
+    MLP_CHECK(h, hipMemcpyAsync(xin, x, sizeof(float) * (size_t)rows * h->n_in, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(ew_blocks((int64_t)rows * h->n_in)), dim3(EW_BLOCK), 0, h->stream, xin,
+        act[0], (int64_t)rows * h->n_in);
+    MLP_CHECK(h, hipGetLastError());
+    const int L = (int)h->dims.size() - 1;
+    const uint16_t* in = act[0];
+    for (int l = 0; l < L; ++l) {
+        const int K = h->dims[l], N = h->dims[l + 1];
+        uint16_t* out = act[1 + (l % 2)];
+        int rc;
+        if ((rc = gemm<false, true>(h, in, h->pbT + h->w_off[l], z, rows, N, K)) != 0) { return rc; }
+        hipLaunchKernelGGL(bias_act_kernel, dim3(ew_blocks((int64_t)rows * N)), dim3(EW_BLOCK), 0, h->stream, z,
+            h->p + h->b_off[l], out, (int64_t)rows, N, l + 1 < L ? 1 : 0);
+        MLP_CHECK(h, hipGetLastError());
+        in = out;
+    }
+    hipLaunchKernelGGL(column0_kernel, dim3(ew_blocks(rows)), dim3(EW_BLOCK), 0, h->stream, z, yd, rows, PAD_OUT);
+    MLP_CHECK(h, hipGetLastError());
+    MLP_CHECK(h, hipMemcpyAsync(y, yd, sizeof(float) * (size_t)rows, hipMemcpyDeviceToHost, h->stream));
+    MLP_CHECK(h, hipStreamSynchronize(h->stream));
     return PSIM_OK;
 }
 
