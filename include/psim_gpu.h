@@ -96,8 +96,23 @@ typedef struct {
     int32_t material;
     int32_t base_table; /* index into psim_tables */
     int32_t scatter_table;
-    int32_t pad_;
+    /* Flight cap of the sensor's model (psim_flight_cap of that model's sensor count); 0 lets
+     * psim_gpu_upload use psim_flight_cap(n_sensors) of the whole upload (one model). */
+    int32_t flight_cap;
 } psim_sensor;
+
+/* Phonons fly across the measurement boundaries (they do not stop at each one); from the
+ * first recorded boundary on, a flight also ends at every flight_cap-th boundary, so that the
+ * engine's LDS tally windows of that length hold whole flights.  The cap of a model depends only on
+ * its number of sensors: the number of steps of tallies (20 B per sensor, one spare row) that fit
+ * 48 KB, at most 128; models with too many sensors for 16 such steps are tallied in HBM and take
+ * 128.  It changes the rounding of positions (one flight is summed instead of several), not the
+ * statistics; a CPU build must use the same rule to reproduce the engine's trajectories. */
+static inline int32_t psim_flight_cap(int32_t n_sensors) {
+    const int32_t s = n_sensors > 0 ? n_sensors : 1;
+    const int32_t w = 49152 / (20 * s) - 1;
+    return (w < 16 || w > 128) ? 128 : w;
+}
 
 typedef struct {
     int32_t n_cells, n_sensors, n_subs, pad_;
@@ -209,9 +224,9 @@ int psim_gpu_last_timing(const psim_gpu_ctx* ctx, double* step_kernel_ms, int64_
     double* run_ms);
 
 /* Engine tuning: measurement steps per transport launch (0 = automatic).  No length changes a
- * result.  Recording launches (LDS tallies) are rounded down to a multiple of 16 steps, so that
- * they hold whole flights (which end at every 16th recorded boundary); one of fewer than 16 steps
- * lets flights pass its end, and those boundaries are tallied into HBM with atomics (slower). */
+ * result.  By default a recording launch (LDS tallies) is as long as the model's flight cap
+ * (psim_flight_cap), so it holds whole flights; a window that does not line up with the caps lets
+ * flights pass its end, and their boundaries past it are tallied in HBM (same sums, slower). */
 int psim_gpu_set_window(psim_gpu_ctx* ctx, int32_t steps_per_launch);
 
 /* Engine tuning: transport blocks per compute unit (default 8). */

@@ -28,6 +28,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "psim_gpu.h" /* psim_flight_cap */
 #include "psim_math.h"
 #include "psim_rng.h"
 
@@ -40,10 +41,6 @@
 #define SCALING_FACTOR 1e9
 #define VELOCITY_EPS 0.01
 #define MAX_COLLISIONS 100
-/* Flights through the recorded steps also end at every FLIGHT_STEPS-th boundary (counted from
- * the first boundary of the recorded stretch), so that a window of a multiple of it holds whole
- * flights; the engine uses the same constant. */
-#define FLIGHT_STEPS 16
 #define TEMP_BOUND_EPS 10.0
 #define PHASOR_TEMP_BOUND_EPS (TEMP_BOUND_EPS * 100.0)
 #define SS_STEPS_PERCENT 0.1
@@ -1354,7 +1351,8 @@ static void tally(const or_model* m, const phonon_t* p, int64_t idx, tally_t* ta
  * velocity), so the phonon does not stop at each boundary.  Its flight ends at its scattering
  * time, at an impact that changes sensor, or at its cap: boundary step_adjustment - 1 in steady
  * state while the step lies below it (the recorded steps start there), else boundary N, where
- * it dies; from `stop` on it also ends at every FLIGHT_STEPS-th boundary after `stop`.  The
+ * it dies; from `stop` on it also ends at every flight_cap-th boundary after `stop`, where
+ * flight_cap = psim_flight_cap(number of sensors) is the engine's rule (psim_gpu.h).  The
  * boundaries a flight passes are handled as events in time order: at each impact
  * before the surface acts (so they see the direction before it and the surface sees the step
  * of the impact) and at the end of the flight (advance_step).  Every draw therefore comes from
@@ -1403,7 +1401,9 @@ static void simulate_phonon(const or_model* m, phonon_t* p, uint64_t seed, uint3
                 if (step < stop) {
                     cap = stop;
                 } else {
-                    cap = stop + FLIGHT_STEPS * ((step - stop) / FLIGHT_STEPS + 1);
+                    /* from `stop` on, every flight_cap-th boundary (psim_flight_cap, psim_gpu.h) */
+                    const int64_t fc = psim_flight_cap(m->n_sensors);
+                    cap = stop + fc * ((step - stop) / fc + 1);
                     if (cap > N) { cap = N; }
                 }
                 time_to_measurement = (double)cap * m->sim_time / (double)N - phonon_age;

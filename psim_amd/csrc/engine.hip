@@ -39,8 +39,7 @@ namespace {
 constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
 constexpr int LDS_TALLY_BYTES = 48 * 1024;  // budget for one block's [W][S] window tally
-constexpr int GLOBAL_TALLY_STEPS = FLIGHT_STEPS;  // fewer LDS-tallied steps per window than this (a window
-                                                   // that cannot hold whole flights): global atomics
+constexpr int GLOBAL_TALLY_STEPS = 16;  // fewer LDS-tallied steps per window than this: HBM tallies
 // Fewer phonons per sensor than this: global atomics too.  Small runs lose more to the many short
 // LDS windows (launches, reductions, sorts) than HBM atomics lose to contention, large ones the
 // reverse (measured at 1e4 and 1e5 phonons per sensor: 1.8-3x faster in HBM below, 2.7x slower
@@ -49,18 +48,23 @@ constexpr int64_t LDS_MIN_PHONONS_PER_SENSOR = 30000;
 constexpr uint32_t QUEUE_CLAIM = 32;  // work-queue items a wave claims per atomic
 constexpr uint32_t OUT_CLAIM = 64;  // survivor slots a wave reserves per atomic
 
-// Tallies of a launch window.  Normally privatised: a [W][S] tally per block in LDS, summed into
-// per-block partials.  When even GLOBAL_TALLY_STEPS steps of S sensors do not fit the LDS budget
-// (thousands of sensors, e.g. a batch of geometries), every tally goes straight to the [S][R]
-// arrays with atomics instead; with that many sensors they rarely collide.
+// Tallies of a launch window.  A boundary a flight passes is recorded as part of a segment: the
+// flight adds its sign and velocity over boundaries a..b, which a difference array stores as two
+// entries (+ at a, - at b + 1) whatever the flight's length.  Normally the segment lies in the
+// window and goes to a [W + 1][S] difference array per block in LDS, summed into per-block
+// partials [W][S] at the block's end.  A segment past the window's end (a flight that crossed it),
+// or every segment when even GLOBAL_TALLY_STEPS steps of S sensors do not fit the LDS budget
+// (thousands of sensors, e.g. a batch of geometries), goes to [S][R + 1] difference arrays in HBM
+// with atomics, summed into the tallies when the run ends (diff_prefix_kernel).
 struct Tally {
     int32_t* __restrict__ pcnt;  // [grid][W*S]
     double* __restrict__ pfx;
     double* __restrict__ pfy;
-    int64_t* __restrict__ counts;  // [S][R] (global mode)
-    double* __restrict__ flux;  // [S][R][2] (global mode)
+    int64_t* __restrict__ dcnt;  // [S][R + 1] (HBM difference arrays)
+    double* __restrict__ dfx;
+    double* __restrict__ dfy;
     int32_t rec;  // 1 if this window records
-    int32_t global;  // 1: atomics into counts / flux
+    int32_t global;  // 1: every segment into the HBM difference arrays
     int32_t W;
 };
 
@@ -309,11 +313,15 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     const uint64_t seed = *seed_p;
     extern __shared__ unsigned char lds_raw[];
     const int S = m.n_sensors;
+    // LDS window tallies as difference arrays over the window's steps: a flight that passes
+    // boundaries a..b adds its count and flux at row a and subtracts them at row b + 1, so its
+    // tallies cost six atomics whatever its length; rows 0..W-1 plus one for b = k1.
     const int WS = (TALLY == TALLY_LDS && tally.rec) ? tally.W * S : 0;
+    const int WS1 = WS ? WS + S : 0;
     double* lfx = reinterpret_cast<double*>(lds_raw);
-    double* lfy = lfx + WS;
-    int32_t* lcnt = reinterpret_cast<int32_t*>(lfy + WS);
-    for (int i = threadIdx.x; i < WS; i += BLOCK) {
+    double* lfy = lfx + WS1;
+    int32_t* lcnt = reinterpret_cast<int32_t*>(lfy + WS1);
+    for (int i = threadIdx.x; i < WS1; i += BLOCK) {
         lfx[i] = 0.;
         lfy[i] = 0.;
         lcnt[i] = 0;
@@ -343,16 +351,18 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     int cur_sensor = 0;  // sensor of p.cell
     psim_stream st;
     psim_stream_init(&st, seed, PSIM_TAG_TRANSPORT, 0, 0);
-    // A flight ends at boundary `stop` (steady state, while the step lies below it), else at the
-    // next multiple of FLIGHT_STEPS boundaries after m.stop, or at boundary N.  Only untallied
-    // launches hold phonons below `stop`: every window that records starts at step adj - 1 = stop
-    // or later.
+    // Only untallied launches hold phonons below `stop`: every window that records starts at step
+    // adj - 1 = stop or later.
     const int32_t stop = TALLY == TALLY_NONE ? m.stop : 0;
     const double steps_per_time = (double)m.N / m.sim_time;
-    auto cap_of = [&](int32_t step) -> int32_t {
+    // A flight ends at boundary `stop` (steady state, while the step lies below it), from there on at
+    // every cap_len-th boundary after it (the model's flight cap), and at boundary N.  `cap` is the
+    // current flight's cap, set when a phonon is fetched and when a flight reaches its cap.
+    int32_t cap_len = 1, cap = 0;
+    auto cap_for = [&](int32_t step) -> int32_t {
         if (step < stop) { return stop; }
-        const int32_t cap = m.stop + FLIGHT_STEPS * ((step - m.stop) / FLIGHT_STEPS + 1);
-        return cap < m.N ? cap : m.N;
+        const int32_t c2 = m.stop + cap_len * ((step - m.stop) / cap_len + 1);
+        return c2 < m.N ? c2 : m.N;
     };
     // Work-queue items the wave has claimed but not handed out yet (wave-uniform): a refill takes
     // them in order and claims QUEUE_CLAIM more with one atomic when they run out.
@@ -363,7 +373,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     // stall bound, S_TTS when a scattering time is due (drawn once, then drift_setup), else
     // drift_setup directly.
     auto drift_setup = [&]() {
-        ttm = (double)cap_of(p.step) * m.sim_time / (double)m.N - age;  // time to the flight's cap
+        ttm = (double)cap * m.sim_time / (double)m.N - age;  // time to the flight's cap
         drift_time = (ttm < tts) ? ttm : tts;
         sensor0 = cur_sensor;
         drifted = 0.;
@@ -382,27 +392,41 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
         }
         drift_setup();
     };
-    // Sensor::updateHeatParams (sensor.cpp:43-52) for boundary k: the window's LDS tallies for
-    // k in (k0, k1], the HBM tallies for a boundary a flight passed beyond the window.
-    auto tally_boundary = [&](int32_t k) {
+    // Sensor::updateHeatParams (sensor.cpp:43-52) for the boundaries a..b a flight passed (a >= adj),
+    // with the sensor and velocity of that moment: the recorded ones (index below R) as one segment
+    // of a difference array, the part in (k0, k1] in the window's LDS, the rest (past k1, after a
+    // flight that crossed the window's end, or all of it in TALLY_GLOBAL) in HBM.
+    auto tally_range = [&](int32_t a, int32_t b) {
         if constexpr (TALLY != TALLY_NONE) {
-            const int idx = k - m.adj;
-            if (idx < m.R && tally.rec) {
-                const double sg = (double)p.sign;
-                if (TALLY == TALLY_GLOBAL || k > k1) {
-                    const int64_t t = (int64_t)cur_sensor * m.R + idx;
-                    atomicAdd(reinterpret_cast<unsigned long long*>(&tally.counts[t]),
-                        (unsigned long long)(int64_t)p.sign);
-                    atomicAdd(&tally.flux[2 * t], p.dx * p.vel * sg);
-                    atomicAdd(&tally.flux[2 * t + 1], p.dy * p.vel * sg);
-                } else {
-                    const int e = (k - k0 - 1) * S + cur_sensor;
-                    atomicAdd(&lcnt[e], p.sign);
-                    atomicAdd(&lfx[e], p.dx * p.vel * sg);
-                    atomicAdd(&lfy[e], p.dy * p.vel * sg);
+            const int32_t last = b < m.adj + m.R - 1 ? b : m.adj + m.R - 1;
+            if (a > last || !tally.rec) { return; }
+            const double sg = (double)p.sign;
+            const double vx = p.dx * p.vel * sg, vy = p.dy * p.vel * sg;
+            int32_t first = a;
+            if constexpr (TALLY == TALLY_LDS) {
+                const int32_t in_last = last < k1 ? last : k1;
+                if (a <= in_last) {
+                    const int e0 = (a - k0 - 1) * S + cur_sensor, e1 = (in_last - k0) * S + cur_sensor;
+                    atomicAdd(&lcnt[e0], p.sign);
+                    atomicAdd(&lcnt[e1], -p.sign);
+                    atomicAdd(&lfx[e0], vx);
+                    atomicAdd(&lfx[e1], -vx);
+                    atomicAdd(&lfy[e0], vy);
+                    atomicAdd(&lfy[e1], -vy);
+                    first = in_last + 1;
                 }
-                c.tallies++;
             }
+            if (first <= last) {
+                const int64_t row = (int64_t)cur_sensor * (m.R + 1) - m.adj;
+                const int64_t t0 = row + first, t1 = row + last + 1;
+                atomicAdd(reinterpret_cast<unsigned long long*>(&tally.dcnt[t0]), (unsigned long long)(int64_t)p.sign);
+                atomicAdd(reinterpret_cast<unsigned long long*>(&tally.dcnt[t1]), (unsigned long long)(int64_t)-p.sign);
+                atomicAdd(&tally.dfx[t0], vx);
+                atomicAdd(&tally.dfx[t1], -vx);
+                atomicAdd(&tally.dfy[t0], vy);
+                atomicAdd(&tally.dfy[t1], -vy);
+            }
+            c.tallies += (uint32_t)(last - a + 1);
         }
     };
     // The measurement events of the boundaries a flight passed up to time t
@@ -416,9 +440,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
         if (est > k) { k = est < cap - 1 ? est : cap - 1; }
         while (k + 1 < cap && (double)(k + 1) * m.sim_time / (double)m.N <= t) { ++k; }
         if (k == p.step) { return false; }
-        if constexpr (TALLY != TALLY_NONE) {
-            for (int32_t j = (p.step + 1 > m.adj ? p.step + 1 : m.adj); j <= k; ++j) { tally_boundary(j); }
-        }
+        if constexpr (TALLY != TALLY_NONE) { tally_range(p.step + 1 > m.adj ? p.step + 1 : m.adj, k); }
         c.steps += (uint32_t)(k - p.step);
         p.step = k;
         events = 0;
@@ -440,7 +462,6 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
         age += drift_time;
         ttm -= drift_time;
         tts -= drift_time;
-        const int32_t cap = cap_of(p.step);
         const bool advanced = advance_step(age, cap);
         if (ttm == 0. && cap == m.N) {  // boundary N: the run is over (step_times_ has N entries)
             c.steps++;
@@ -452,11 +473,12 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
         if (ttm == 0.) {  // the flight reached boundary `cap`
             c.steps++;
             if constexpr (TALLY != TALLY_NONE) {
-                if (cap >= m.adj) { tally_boundary(cap); }
+                if (cap >= m.adj) { tally_range(cap, cap); }
             }
             p.step = cap;
             events = 0;
             psim_stream_set_step(&st, (uint32_t)cap);
+            cap = cap_for(p.step);
             if (p.step >= k1) {
                 state = S_PAUSED;
                 return;
@@ -540,6 +562,8 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                     p.step = (int32_t)(age / m.step_time);
                 }
                 if (!MULTI) { sid = slot; }
+                cap_len = m.sensors[cur_sensor].flight_cap;
+                cap = cap_for(p.step);
                 p.px = pos.x;
                 p.py = pos.y;
                 p.dx = dir.x;
@@ -580,7 +604,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
             // the boundaries passed before the impact see the old direction; the surface sees
             // the step of the moment of impact
             // the reference counts collisions per drift, and a drift ends at a boundary
-            if (advance_step(age + (drifted + t_hit), cap_of(p.step))) { collisions = 0; }
+            if (advance_step(age + (drifted + t_hit), cap)) { collisions = 0; }
             handle_surface_collision(m, p, p.px, p.py, st);
             if (p.cell < 0) {
                 c.absorbed++;
@@ -718,13 +742,20 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
         const unsigned long long sum = wave_sum64(v[i]);
         if (lane == 0 && sum) { atomicAdd(&counters[slots[i]], sum); }
     }
-    if (WS) {
+    if (WS) {  // the difference arrays summed over the window's steps, per sensor, into the partials
         __syncthreads();
         const int64_t base = (int64_t)blockIdx.x * WS;
-        for (int i = threadIdx.x; i < WS; i += BLOCK) {
-            tally.pcnt[base + i] = lcnt[i];
-            tally.pfx[base + i] = lfx[i];
-            tally.pfy[base + i] = lfy[i];
+        for (int sn = threadIdx.x; sn < S; sn += BLOCK) {
+            int32_t cnt = 0;
+            double fx = 0., fy = 0.;
+            for (int e = sn; e < WS; e += S) {
+                cnt += lcnt[e];
+                fx += lfx[e];
+                fy += lfy[e];
+                tally.pcnt[base + e] = cnt;
+                tally.pfx[base + e] = fx;
+                tally.pfy[base + e] = fy;
+            }
         }
     }
 }
@@ -825,6 +856,27 @@ __global__ void __launch_bounds__(SORT_BLOCK) cell_scatter_kernel(Live src, Live
     }
 }
 
+// Adds the HBM difference arrays [S][R + 1] of a run, summed along the recorded steps, into the
+// tallies [S][R] (one thread per sensor, steps in order).
+__global__ void __launch_bounds__(BLOCK) diff_prefix_kernel(const int64_t* __restrict__ dcnt,
+    const double* __restrict__ dfx, const double* __restrict__ dfy, int S, int64_t R, int64_t* __restrict__ counts,
+    double* __restrict__ flux) {
+    const int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= S) { return; }
+    int64_t cnt = 0;
+    double fx = 0., fy = 0.;
+    for (int64_t r = 0; r < R; ++r) {
+        const int64_t d = (int64_t)s * (R + 1) + r;
+        cnt += dcnt[d];
+        fx += dfx[d];
+        fy += dfy[d];
+        const int64_t t = (int64_t)s * R + r;
+        counts[t] += cnt;
+        flux[2 * t] += fx;
+        flux[2 * t + 1] += fy;
+    }
+}
+
 // Clears n 32-bit words (used instead of memset nodes, so that a run can be captured in a graph).
 __global__ void __launch_bounds__(BLOCK) zero_words_kernel(uint32_t* __restrict__ p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) { p[i] = 0u; }
@@ -895,6 +947,10 @@ struct psim_gpu_ctx {
     int64_t pool_cap = 0;
     int64_t* d_counts = nullptr;
     double* d_flux = nullptr;
+    // HBM difference arrays [S][R + 1] of the tally segments that do not go to LDS
+    int64_t* d_dcnt = nullptr;
+    double* d_dfx = nullptr;
+    double* d_dfy = nullptr;
     unsigned long long* d_counters = nullptr;
     int32_t* d_pcnt = nullptr;
     double* d_pfx = nullptr;
@@ -994,6 +1050,18 @@ void free_steps(psim_gpu_ctx* c) {
     c->steps_cap = 0;
 }
 
+void free_tallies(psim_gpu_ctx* c) {
+    void* ps[] = { c->d_counts, c->d_flux, c->d_dcnt, c->d_dfx, c->d_dfy };
+    for (void* p : ps) {
+        if (p) { (void)hipFree(p); }
+    }
+    c->d_counts = nullptr;
+    c->d_flux = nullptr;
+    c->d_dcnt = nullptr;
+    c->d_dfx = nullptr;
+    c->d_dfy = nullptr;
+}
+
 void free_partials(psim_gpu_ctx* c) {
     if (c->d_pcnt) { (void)hipFree(c->d_pcnt); }
     if (c->d_pfx) { (void)hipFree(c->d_pfx); }
@@ -1023,11 +1091,9 @@ int upload_array(psim_gpu_ctx* c, const T* host, size_t n, const T** out) {
 constexpr int32_t DEFAULT_FREE_WINDOW = 1 << 30;
 
 int32_t record_window_for(const psim_gpu_ctx* c) {
-    const int64_t by_lds = std::max<int64_t>(1, LDS_TALLY_BYTES / (20 * std::max(1, c->n_sensors)));
-    const int64_t w = std::min<int64_t>(c->window > 0 ? c->window : 128, by_lds);
-    // A multiple of FLIGHT_STEPS holds whole flights: no tally spills past the window into HBM.
-    // Shorter windows let flights pass their end (same results, HBM atomics for those steps).
-    return (int32_t)(w >= FLIGHT_STEPS ? w - w % FLIGHT_STEPS : w);
+    // (W + 1) rows of 20 B per sensor: the difference arrays need one row past the window
+    const int64_t by_lds = std::max<int64_t>(1, LDS_TALLY_BYTES / (20 * std::max(1, c->n_sensors)) - 1);
+    return (int32_t)std::min<int64_t>(c->window > 0 ? c->window : 128, by_lds);
 }
 
 int32_t free_window_for(const psim_gpu_ctx* c) { return c->window > 0 ? c->window : DEFAULT_FREE_WINDOW; }
@@ -1037,7 +1103,7 @@ int32_t free_window_for(const psim_gpu_ctx* c) { return c->window > 0 ? c->windo
 // whole run).  psim_gpu_set_tally_mode forces either (LDS only where one step fits).
 bool use_global_tally(const psim_gpu_ctx* c) {
     const int64_t sensors = std::max(1, c->n_sensors);
-    const int64_t lds_steps = LDS_TALLY_BYTES / (20 * sensors);
+    const int64_t lds_steps = LDS_TALLY_BYTES / (20 * sensors) - 1;
     if (c->tally_mode == PSIM_TALLY_HBM || lds_steps < 1) { return true; }
     if (c->tally_mode == PSIM_TALLY_LDS) { return false; }
     return lds_steps < GLOBAL_TALLY_STEPS || c->params.total_phonons < LDS_MIN_PHONONS_PER_SENSOR * sensors;
@@ -1242,6 +1308,10 @@ int enqueue_run(psim_gpu_ctx* c, const RunPlan& plan, bool eager) {
     int rc;
     if ((rc = zero_fill(c, c->d_counts, sizeof(int64_t) * (size_t)std::max<int64_t>(1, S * R))) != 0) { return rc; }
     if ((rc = zero_fill(c, c->d_flux, sizeof(double) * (size_t)std::max<int64_t>(1, 2 * S * R))) != 0) { return rc; }
+    const size_t SR1 = (size_t)S * (size_t)(R + 1);
+    if ((rc = zero_fill(c, c->d_dcnt, sizeof(int64_t) * SR1)) != 0) { return rc; }
+    if ((rc = zero_fill(c, c->d_dfx, sizeof(double) * SR1)) != 0) { return rc; }
+    if ((rc = zero_fill(c, c->d_dfy, sizeof(double) * SR1)) != 0) { return rc; }
     if ((rc = zero_fill(c, c->d_counters, sizeof(unsigned long long) * PSIM_NUM_COUNTERS)) != 0) { return rc; }
     if ((rc = zero_fill(c, c->d_birth_hist, sizeof(uint32_t) * (size_t)plan.steps_need)) != 0) { return rc; }
     if ((rc = zero_fill(c, c->d_n_live, sizeof(uint32_t) * (size_t)plan.steps_need)) != 0) { return rc; }
@@ -1266,8 +1336,8 @@ int enqueue_run(psim_gpu_ctx* c, const RunPlan& plan, bool eager) {
             c->d_born_offset, w };
         // window steps k0+1 .. k1 map to tally indices (k - adj) in [0, R)
         const bool rec = (k1 >= P.step_adjustment) && (k0 + 1 - P.step_adjustment < R) && (k1 - k0 <= W);
-        Tally t{ c->d_pcnt, c->d_pfx, c->d_pfy, c->d_counts, c->d_flux, rec ? 1 : 0, global_tally ? 1 : 0, W };
-        const size_t lds = (rec && !global_tally) ? (size_t)WS * 20 : 0;
+        Tally t{ c->d_pcnt, c->d_pfx, c->d_pfy, c->d_dcnt, c->d_dfx, c->d_dfy, rec ? 1 : 0, global_tally ? 1 : 0, W };
+        const size_t lds = (rec && !global_tally) ? (size_t)(WS + S) * 20 : 0;
         if (eager) { HIP_CHECK(c, hipEventRecord(c->ev[1], c->stream)); }
         launch_transport(c, !rec ? TALLY_NONE : global_tally ? TALLY_GLOBAL : TALLY_LDS, grid, !rec ? 0 : lds, lists,
             (int32_t)k0, (int32_t)k1, t, fate);
@@ -1311,6 +1381,11 @@ int enqueue_run(psim_gpu_ctx* c, const RunPlan& plan, bool eager) {
                 (long long)k0, (long long)k1, (double)ms);
             std::fflush(stderr);
         }
+    }
+    if (n > 0 && R > 0) {  // the segments tallied in HBM
+        hipLaunchKernelGGL(diff_prefix_kernel, dim3((unsigned)((S + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, c->stream,
+            c->d_dcnt, c->d_dfx, c->d_dfy, S, R, c->d_counts, c->d_flux);
+        HIP_CHECK(c, hipGetLastError());
     }
     return PSIM_OK;
 }
@@ -1440,8 +1515,7 @@ void psim_gpu_destroy(psim_gpu_ctx* c) {
     free_partials(c);
     free_lists(c);
     free_steps(c);
-    if (c->d_counts) { (void)hipFree(c->d_counts); }
-    if (c->d_flux) { (void)hipFree(c->d_flux); }
+    free_tallies(c);
     if (c->d_counters) { (void)hipFree(c->d_counters); }
     if (c->d_seed) { (void)hipFree(c->d_seed); }
     if (c->h_seed) { (void)hipHostFree(c->h_seed); }
@@ -1504,6 +1578,7 @@ int psim_gpu_upload(psim_gpu_ctx* c, const psim_geom* g, const psim_tables* t, c
             || ss.scatter_table < 0 || ss.scatter_table >= t->n_tables) {
             return fail(c, PSIM_E_INVALID, "sensor material or table out of range");
         }
+        if (ss.flight_cap < 0) { return fail(c, PSIM_E_INVALID, "negative flight cap"); }
     }
     int64_t expect = 0;
     bool multi = false;
@@ -1534,7 +1609,11 @@ int psim_gpu_upload(psim_gpu_ctx* c, const psim_geom* g, const psim_tables* t, c
     PSIM_UPLOAD(cells, g->cells, g->n_cells);
     PSIM_UPLOAD(edges, g->edges, (size_t)g->n_cells * 3);
     PSIM_UPLOAD(subs, g->subs, g->n_subs);
-    PSIM_UPLOAD(sensors, g->sensors, g->n_sensors);
+    std::vector<psim_sensor> sensors(g->sensors, g->sensors + g->n_sensors);
+    for (psim_sensor& se : sensors) {  // 0: the cap of one model with every sensor of the upload
+        if (se.flight_cap == 0) { se.flight_cap = psim_flight_cap(g->n_sensors); }
+    }
+    PSIM_UPLOAD(sensors, sensors.data(), g->n_sensors);
     PSIM_UPLOAD(mats, t->materials, t->n_materials);
     PSIM_UPLOAD(freq, t->freq, nb * t->n_materials);
     PSIM_UPLOAD(vel_la, t->vel_la, nb * t->n_materials);
@@ -1578,19 +1657,17 @@ int psim_gpu_upload(psim_gpu_ctx* c, const psim_geom* g, const psim_tables* t, c
     c->m = m;
     c->params = *p;
     const int64_t SR = (int64_t)g->n_sensors * p->num_record;
-    if (c->n_sensors != g->n_sensors || !c->d_counts) {
-        if (c->d_counts) { (void)hipFree(c->d_counts); }
-        if (c->d_flux) { (void)hipFree(c->d_flux); }
-        c->d_counts = nullptr;
-        c->d_flux = nullptr;
-    }
-    if (c->d_counts) {
-        (void)hipFree(c->d_counts);
-        (void)hipFree(c->d_flux);
-    }
+    const int64_t SR1 = (int64_t)g->n_sensors * (p->num_record + 1);
+    free_tallies(c);
     c->d_counts = dalloc<int64_t>((size_t)std::max<int64_t>(1, SR));
     c->d_flux = dalloc<double>((size_t)std::max<int64_t>(1, 2 * SR));
-    if (!c->d_counts || !c->d_flux) { return fail(c, PSIM_E_DEVICE, "hipMalloc failed for tallies"); }
+    c->d_dcnt = dalloc<int64_t>((size_t)SR1);
+    c->d_dfx = dalloc<double>((size_t)SR1);
+    c->d_dfy = dalloc<double>((size_t)SR1);
+    if (!c->d_counts || !c->d_flux || !c->d_dcnt || !c->d_dfx || !c->d_dfy) {
+        free_tallies(c);
+        return fail(c, PSIM_E_DEVICE, "hipMalloc failed for tallies");
+    }
     c->n_sensors = g->n_sensors;
     c->n_cells = g->n_cells;
     c->uploaded = true;

@@ -41,11 +41,6 @@ constexpr uint32_t META_NEG = 1u << 11;
 constexpr uint32_t META_STARTED = 1u << 12;
 constexpr int META_STREAM_SHIFT = 16;
 
-// Flights through the recorded steps also end at every FLIGHT_STEPS-th boundary after the model's
-// `stop` boundary, so that a window of a multiple of it holds whole flights (the oracle uses the
-// same constant).
-constexpr int32_t FLIGHT_STEPS = 16;
-
 struct Model {
     const psim_cell* __restrict__ cells;
     const psim_edge* __restrict__ edges;

@@ -444,6 +444,8 @@ void GpuModelSimulator::initPhononBuilders(std::vector<Cell>& cells, std::vector
         d.material = mat_index(s.getMaterial());
         d.base_table = table_index(s.getMaterial(), 0, s.tableIndex());
         d.scatter_table = table_index(s.getMaterial(), 2, s.tableIndex());
+        // the model's own cap, which a batch keeps for this model's phonons
+        d.flight_cap = psim_flight_cap(static_cast<int32_t>(sensors.size()));
     }
     std::vector<psim_cell> d_cells(cells.size());
     std::vector<psim_edge> d_edges(cells.size() * 3);

@@ -91,12 +91,11 @@ def test_trajectories_match_oracle(engine, name, tally):
     np.testing.assert_allclose(res.measurements[:, 2:], meas[:, 2:], rtol=1e-9, atol=1e-9 * np.max(np.abs(meas[:, 2:])))
 
 
-@pytest.mark.parametrize("sensors,length,launches", [(10, 1000.0, 8), (40, 4000.0, 21)])
-def test_recording_windows_hold_whole_flights(engine, sensors, length, launches):
-    """Default LDS recording windows are sized by the 48 KB budget (20 B per sensor and step), at
-    most 128 steps, and rounded down to a multiple of the 16-step flight cap, so no flight passes
-    a window's end: 10 sensors give 128-step windows (1000 steps in 8 launches), 40 sensors 61
-    rounded to 48 (21 launches)."""
+@pytest.mark.parametrize("sensors,length,launches", [(10, 1000.0, 8), (40, 4000.0, 17)])
+def test_recording_windows_fill_the_lds_budget(engine, sensors, length, launches):
+    """Default LDS recording windows are sized by the 48 KB budget: W + 1 rows of 20 B per sensor
+    (the difference arrays of the window's tallies), at most 128 steps.  10 sensors give 128-step
+    windows (1000 steps in 8 launches), 40 sensors 48 KB / 800 B - 1 = 60 steps (17 launches)."""
     cfg = configs.linear_bar(num_sensors=sensors, length=length, num_phonons=20000, num_measurements=1000,
                              sim_time=5.0, spec=0.5, sim_type=1, step_interval=50).to_dict()
     print(f"case {sensors} sensors", flush=True)

@@ -326,8 +326,8 @@ def test_flights_agree_with_the_step_by_step_loop(name):
     # Where the tallies land: the per-sensor, per-step counts of the two loops.  The trajectories
     # that rounding sends elsewhere move at most 0.4 % of the absolute tally mass (the L-shape,
     # whose phonons live ~1200 steps); a flight that tallies its passed boundaries one step late
-    # or into the next sensor moves 9-180 %.  Both planted mistakes (an oracle test hook) must
-    # fail the same bound.
+    # moves 2.8-4.2 %, one that tallies them into the next sensor 9-190 %.  Both planted mistakes
+    # (an oracle test hook) must exceed 2 %, and the real loop stay below 1 %.
     def moved(x):
         return np.abs(x.counts - b.counts).sum() / np.abs(b.counts).sum()
 
@@ -335,7 +335,7 @@ def test_flights_agree_with_the_step_by_step_loop(name):
     for fault in (1, 2):
         bad = oracle.Model(fix["model"])
         bad.plant_flight_fault(fault)
-        assert moved(bad.run(0, nthreads=8)) > 5e-2, fault
+        assert moved(bad.run(0, nthreads=8)) > 2e-2, fault
 
 
 STATS_ONLY = ["si_ge", "phasor", "needles", "kinked_15", "kinked_90"]
