@@ -67,7 +67,7 @@ sys.path.insert(0, ROOT)
 BYTES_PER_PHONON_STEP = 120.0
 HBM_PEAK_GBPS = 8000.0
 ENGINE_SOURCES = ("psim_amd/csrc/engine.hip", "psim_amd/csrc/engine_device.hpp", "psim_amd/csrc/psim_rng.h",
-                  "psim_amd/csrc/psim_math.h")
+                  "psim_amd/csrc/psim_math.h", "include/psim_gpu.h")
 
 
 def engine_sha256() -> str:
@@ -507,11 +507,10 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
-    # tally atomics: every tallied phonon-step adds its count (int32) and two flux components (FP64)
-    # with LDS atomics (or HBM atomics when the window tallies do not fit LDS)
-    adds = 3 * counters["tallies"]
-    line["tally_atomics"] = {"adds_per_run": adds,
-                             "adds_per_s_of_transport_time": adds / (kernel_ms / 1e3 / args.steps)}
+    # tallies: boundaries recorded per run and per second of transport time; a flight records the
+    # boundaries it passes as one difference-array segment (six atomics whatever its length)
+    line["tallies"] = {"boundaries_per_run": counters["tallies"],
+                       "boundaries_per_s_of_transport_time": counters["tallies"] / (kernel_ms / 1e3 / args.steps)}
     # What the counters say of this kernel on this workload (rocprofv3 --pmc, profiles/): HBM bytes
     # per phonon-step (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and VALU
     # issue.  Counters cannot be read inside this process, so these figures are copied from the
