@@ -4,6 +4,7 @@ and measures the warm start on held-out structures.
   python -m psim_amd.train_surrogate data     --out DIR [--first 0] [--count 20000] [--phonons 100000]
   python -m psim_amd.train_surrogate train    --out DIR [--epochs 40]      (writes DIR/surrogate.npz)
   python -m psim_amd.train_surrogate evaluate --out DIR [--weights FILE] [--count 512] [--phonons 8000000]
+                                              [--max-wall-diff 10]   (absolute 0.05 K criterion)
   python -m psim_amd.train_surrogate cost     --out DIR [--weights FILE]   (GPU seconds, from DIR/evaluation.json)
 
 Training structures are psim_amd.configs.batch_geometry_params(random.Random(TRAIN_SEED + i)),
@@ -11,8 +12,8 @@ held-out ones use EVAL_SEED + i; the two sets share no structure.  Targets are t
 steady-state shape u of every sensor (psim_amd.surrogate.steady_state_targets).  The evaluation runs every held-out structure from three starts (cold: every
 sensor at t_eq; linear: linear in arc length between the walls; warm: the network's prediction)
 over 25 ns in 5000 steps of 5 ps, and reports the time to steady state of each: the RMS over
-sensors of the change in 100-step (0.5 ns) window-mean temperature below rel_tol x |t_hot - t_cold|
-for three windows in a row.
+sensors of the change in 100-step (0.5 ns) window-mean temperature below rel_tol x |t_hot - t_cold|,
+or below the absolute 0.05 K, for three windows in a row.
 """
 from __future__ import annotations
 
@@ -30,16 +31,26 @@ EVAL_SEED = 2_000_000
 MIN_WALL_DIFF = 0.5  # K: structures with nearly equal walls carry no signal
 REL_TOLS = (0.0025, 0.004, 0.005, 0.0075, 0.01)
 HEADLINE_TOL = 0.005
+# The absolute criterion of the benchmark: RMS change below 0.05 K.  The deviational noise floor of a
+# 100-step window mean grows with the wall difference (about 0.0034 x |t_hot - t_cold| at 4e6
+# phonons), so it is met at affordable phonon counts on structures whose walls differ by at most
+# ABS_MAX_WALL_DIFF (0.024 K at 8e6 phonons and 10 K, half the tolerance).
+ABS_TOL_K = 0.05
+ABS_KEY = "abs0.05"
+ABS_MAX_WALL_DIFF = 10.0
 
 
-def structures(seed0: int, count: int) -> list:
+def structures(seed0: int, count: int, max_wall_diff: float | None = None) -> list:
+    """The first `count` structures of the family from seed0 on whose walls differ by at least
+    MIN_WALL_DIFF (and at most max_wall_diff, when given)."""
     from . import configs
 
     out, i = [], 0
     while len(out) < count:
         p = configs.batch_geometry_params(random.Random(seed0 + i))
         i += 1
-        if abs(p["t_hot"] - p["t_cold"]) >= MIN_WALL_DIFF:
+        d = abs(p["t_hot"] - p["t_cold"])
+        if d >= MIN_WALL_DIFF and (max_wall_diff is None or d <= max_wall_diff):
             out.append(p)
     return out
 
@@ -129,17 +140,19 @@ def cmd_train(args) -> None:
 
 
 def relaxation_summary(p: dict, temps: np.ndarray, late: int) -> dict:
-    from . import surrogate
+    from . import analysis, surrogate
 
-    return {"ttss": {str(c): surrogate.time_to_steady_state(p, temps, c) for c in REL_TOLS},
-            "late_u": surrogate.shape_of(p, temps[:, -late:].mean(axis=1)).tolist()}
+    ttss = {str(c): surrogate.time_to_steady_state(p, temps, c) for c in REL_TOLS}
+    ttss[ABS_KEY] = analysis.time_to_steady_state(temps, 100, ABS_TOL_K, 3)
+    return {"ttss": ttss, "late_u": surrogate.shape_of(p, temps[:, -late:].mean(axis=1)).tolist()}
 
 
 def evaluate(engine, net, params: list, phonons: int, steps: int, sim_ns: float, seed: int, linear_count: int,
-             progress=log, cost: bool = True) -> dict:
-    """Time to steady state of `params` from the cold, linear and warm starts; the summary and
-    per-structure rows.  With `cost`, also the GPU seconds of the cold and warm runs cut at their
-    times to steady state (HEADLINE_TOL)."""
+             progress=log, cost: bool = True, headline: str = str(HEADLINE_TOL)) -> dict:
+    """Time to steady state of `params` from the cold, linear and warm starts, for every relative
+    tolerance and the absolute 0.05 K (key ABS_KEY); the summary and per-structure rows.  With
+    `cost`, also the GPU seconds of the cold and warm runs cut at their times to steady state under
+    the `headline` criterion."""
     from . import surrogate
 
     late = steps // 5
@@ -163,8 +176,7 @@ def evaluate(engine, net, params: list, phonons: int, steps: int, sim_ns: float,
     dt_ns = sim_ns / steps
     summary = {"structures": len(params), "phonons": phonons, "steps": steps, "dt_ps": dt_ns * 1e3,
                "window_steps": 100, "predict_s": predict_s, "criteria": {}}
-    for c in REL_TOLS:
-        key = str(c)
+    for key in [str(c) for c in REL_TOLS] + [ABS_KEY]:
         row = {}
         for name, res in runs.items():
             t = [r["ttss"][key] for r in res]
@@ -204,8 +216,8 @@ def evaluate(engine, net, params: list, phonons: int, steps: int, sim_ns: float,
                                 / 2.0 for p, c, w in zip(params, runs["cold"], runs["warm"])])
     summary["late_field_warm_minus_cold_rmse_K"] = float(np.sqrt(np.mean(late_diff ** 2)))
     if cost:
-        key = str(HEADLINE_TOL)
-        summary["gpu_seconds_to_steady_state"] = {"criterion": HEADLINE_TOL}
+        key = headline
+        summary["gpu_seconds_to_steady_state"] = {"criterion": headline}
         for name, inits in (("cold", None), ("warm", warm_inits)):
             t1 = time.perf_counter()
             c = surrogate.gpu_seconds_to_steady_state(engine, params, inits, [r["ttss"][key] for r in runs[name]],
@@ -226,10 +238,12 @@ def cmd_evaluate(args) -> None:
     import psim_amd
     from . import surrogate
 
-    params = structures(EVAL_SEED, args.count)
+    params = structures(EVAL_SEED, args.count, args.max_wall_diff)
     engine = psim_amd.Engine(args.device)
     net = surrogate.load_weights(args.weights or surrogate.WEIGHTS_PATH, args.device)
-    res = evaluate(engine, net, params, args.phonons, 5000, 25.0, seed=7, linear_count=args.linear_count)
+    headline = ABS_KEY if args.max_wall_diff is not None else str(HEADLINE_TOL)
+    res = evaluate(engine, net, params, args.phonons, args.steps, args.sim_ns, seed=7, linear_count=args.linear_count,
+                   headline=headline)
     os.makedirs(args.out, exist_ok=True)
     with open(os.path.join(args.out, "evaluation.json"), "w") as f:
         json.dump(res, f, indent=1)
@@ -276,6 +290,11 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--weights", default=None)
     ap.add_argument("--linear-count", type=int, default=128)
+    ap.add_argument("--steps", type=int, default=5000, help="evaluate: measurement steps over --sim-ns")
+    ap.add_argument("--sim-ns", type=float, default=25.0)
+    ap.add_argument("--max-wall-diff", type=float, default=None,
+                    help="evaluate: only held-out structures whose walls differ by at most this many K, with the "
+                         "absolute 0.05 K criterion as the headline (ABS_MAX_WALL_DIFF keeps its noise floor below)")
     args = ap.parse_args(argv)
     if args.command == "data":
         args.count = args.count or 20000

@@ -83,9 +83,11 @@ def _reference_gradients(net, params, x, t):
 
 
 @pytest.mark.parametrize("rows", [150, 2100])
-def test_gradients_match_cpu_reference(surrogate, rows):
-    """At 2100 rows the weight gradients (K = rows) are split over K."""
-    net = surrogate.Surrogate(n_in=32, hidden=96, layers=3, seed=1)
+@pytest.mark.parametrize("hidden,layers", [(96, 3), (512, 4)])
+def test_gradients_match_cpu_reference(surrogate, rows, hidden, layers):
+    """At 2100 rows the weight gradients (K = rows) are split over K.  512 x 4 is the warm-start
+    network's own shape (32 -> 512 x 4 -> 1)."""
+    net = surrogate.Surrogate(n_in=32, hidden=hidden, layers=layers, seed=1)
     rng = np.random.default_rng(5)
     x = rng.normal(size=(rows, 32)).astype(np.float32)
     t = rng.normal(size=rows).astype(np.float32)

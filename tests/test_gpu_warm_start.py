@@ -34,6 +34,33 @@ def test_warm_start_cuts_time_to_steady_state(setup):
     assert s["initial_error_rmse_K"]["warm"] < 0.25 * s["initial_error_rmse_K"]["cold"]
 
 
+def test_warm_start_at_the_absolute_criterion(setup):
+    """The benchmark's criterion itself: RMS over sensors of the change in 100-step window-mean T
+    below 0.05 K for 3 windows.  Held-out structures whose walls differ by at most 20 K (where 3.2e7
+    phonons keep the window-mean noise below the tolerance), each from a cold start, the
+    linear-in-arc-length guess and the network's prediction.  On 64 such structures the warm start
+    reaches steady state 3.06x sooner in simulated time and 2.73x sooner in GPU time
+    (profiles/surrogate_warm_start_abs.md); this test runs 16 of them (2.29x and 1.69x with seed 13:
+    the window-mean noise is a few hundredths of a kelvin, so one structure that lingers near the
+    tolerance moves a 16-structure total).  At this tolerance the linear guess, 0.2-0.3 K RMS from the
+    settled field, also settles within a few windows: the network's edge over it is its 4-5x smaller
+    initial error, not its time."""
+    engine, net, surrogate, ts = setup
+    params = ts.structures(ts.EVAL_SEED, 16, max_wall_diff=20.0)
+    res = ts.evaluate(engine, net, params, phonons=32_000_000, steps=5000, sim_ns=25.0, seed=13,
+                      linear_count=len(params), headline=ts.ABS_KEY, progress=lambda m: print(m, flush=True))
+    s = res["summary"]
+    row = s["criteria"][ts.ABS_KEY]
+    print({k: row[k] for k in ("cold", "warm", "linear", "cold_over_warm_total")}, s["gpu_seconds_to_steady_state"],
+          s["initial_error_rmse_K"], flush=True)
+    assert row["warm"]["reached"] == len(params) and row["cold"]["reached"] == len(params)
+    assert row["cold_over_warm_total"] >= 2.0
+    assert s["gpu_seconds_to_steady_state"]["cold_over_warm"] >= 1.4
+    # the network starts closer to the settled field than the guess without it
+    err = s["initial_error_rmse_K"]
+    assert err["warm"] < 0.5 * err["linear"] < 0.5 * err["cold"]
+
+
 def test_warm_start_settles_where_the_reference_iteration_goes(setup):
     """Sensor tables follow the initial temperature (sensorController.cpp:38-50), so the field a run
     settles to depends on its start; the reference iterates toward tables at the steady temperature
