@@ -104,11 +104,14 @@ void dump_table(std::ostream& os, const TableData& t) {
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        std::cerr << "usage: psim_ref <model.txt> <out.json> [--tables T1,T2,...] [--geometry]\n";
+        std::cerr << "usage: psim_ref <model.txt> <out.json> [--tables T1,T2,...] [--geometry] "
+                     "[--export <model.json> <seconds>]\n";
         return 2;
     }
     std::vector<double> table_temps;
     bool dump_geometry = false;
+    std::string export_path;  // Model::exportResults' file argument (model.cpp:184-186)
+    double export_time = 0.;
     for (int i = 3; i < argc; ++i) {
         if (std::strcmp(argv[i], "--tables") == 0 && i + 1 < argc) {
             std::stringstream ss(argv[++i]);
@@ -116,6 +119,9 @@ int main(int argc, char** argv) {
             while (std::getline(ss, tok, ',')) table_temps.push_back(std::stod(tok));
         } else if (std::strcmp(argv[i], "--geometry") == 0) {
             dump_geometry = true;
+        } else if (std::strcmp(argv[i], "--export") == 0 && i + 2 < argc) {
+            export_path = argv[++i];
+            export_time = std::stod(argv[++i]);
         }
     }
     std::ifstream in(argv[1]);
@@ -178,6 +184,8 @@ int main(int argc, char** argv) {
         const auto t0 = std::chrono::steady_clock::now();
         model.runSimulation();
         const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        // The reference's own writer (outputManager.cpp:13-38), as main.cpp:21 calls it.
+        if (!export_path.empty()) model.exportResults(export_path, export_time);
 
         // The budget's energy per phonon: recompute the total energy with every controller's
         // t_steady_ back at t_init (the state before resetRequired ran).
