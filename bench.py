@@ -45,11 +45,15 @@ With --workload surrogate it measures the warm start: held-out structures
 prediction (8e6 phonons, 5000 steps of 5 ps, periodic mode); `value` is the ratio of their total
 times to steady state in simulated time (RMS over sensors of the change in 100-step window-mean T
 below 0.005 x |t_hot - t_cold| for 3 windows); cold_over_warm_gpu_seconds is the same ratio in GPU
-time, from runs cut at their measured steady state.  The roofline is the surrogate's BF16 MFMA GEMM kernel against the
-dense bf16 peak, timed with HIP events; its CPU baseline is the reference's rate on two of the
-held-out structures.
+time, from runs cut at their measured steady state.  The roofline is the surrogate's fused BF16 MFMA forward product
+(gemm_nt_kernel, 65536 x 512 x 512) against the dense bf16 peak, timed with HIP events; its CPU
+baseline is the reference's rate on two of the held-out structures.
 
-Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload lshape|film|batch|surrogate]
+With --workload gemm it measures the surrogate's products alone: the network's fused products of a
+hidden layer at 4096 and 65536 rows (psim_mlp_time_product), hipBLASLt's plain product beside them
+and a training step at batch 4096.
+
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload lshape|film|batch|surrogate|gemm]
 """
 from __future__ import annotations
 
@@ -332,12 +336,101 @@ def batch_bench(args) -> dict:
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
+# HBM bytes of one 65536 x 512 x 512 fused forward product (128 x 128 tile, 8 waves), copied from the
+# counter profile profiles/gemm_nt_kernel_surrogate.md: FETCH_SIZE 39469 KiB x 1.9911 + WRITE_SIZE
+# 83280 KiB x 0.9662 (copy-kernel calibration) per dispatch; counters cannot be read in this process
+GEMM_FWD_65536_TRAFFIC_BYTES = 162.9e6
+
+
+def hipblaslt_gemm(M: int, N: int, K: int) -> dict:
+    """torch.matmul (hipBLASLt) on bf16 [M][K] x [K][N] with a bf16 result, HIP events over 50 products."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    a = torch.randn(M, K, device=dev, dtype=torch.bfloat16)
+    b = torch.randn(K, N, device=dev, dtype=torch.bfloat16)
+    for _ in range(3):
+        a @ b
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(50):
+        a @ b
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 50
+    return {"M": M, "N": N, "K": K, "product": "torch.matmul bf16 (hipBLASLt)", "ms": ms,
+            "tflops": 2.0 * M * N * K / (ms / 1e3) / 1e12}
+
+
+def surrogate_products(rows: int = 0) -> tuple:
+    """The network's own fused products of a hidden 512 x 512 layer (psim_mlp_time_product), at the
+    training batch (4096 rows), at `rows` rows (one held-out forward pass, when given) and at 65536
+    rows: forward (bias, ReLU and bf16 rounding in the epilogue), forward while training (plus the
+    transposed copy), weight gradient (on the transposed copies, split over the rows) and input
+    gradient (through the ReLU); hipBLASLt on the plain 65536-row product beside them; and one
+    training step (forward, backward, Adam) at batch 4096, timed over 50 steps."""
+    import numpy as np
+    from psim_amd import surrogate
+
+    trainer = surrogate.Surrogate(seed=1)
+    shapes = [("forward_hidden_b4096", "forward_train", 4096), ("weight_grad_hidden_b4096", "weight_grad", 4096),
+              ("input_grad_hidden_b4096", "input_grad", 4096)]
+    if rows:
+        shapes.append((f"forward_hidden_{rows}_rows", "forward", rows))
+    shapes += [("forward_train_hidden_65536_rows", "forward_train", 65536),
+               ("weight_grad_hidden_65536_rows", "weight_grad", 65536),
+               ("forward_hidden_65536_rows", "forward", 65536)]
+    gemm = {}
+    for name, kind, r in shapes:
+        ms, tile = trainer.time_product(kind, r, reps=50)
+        M, N, K = (512, 512, r) if kind == "weight_grad" else (r, 512, 512)
+        gemm[name] = {"M": M, "N": N, "K": K, "product": kind, "tile": tile, "ms": ms,
+                      "tflops": 2.0 * M * N * K / (ms / 1e3) / 1e12}
+    gemm["hipblaslt_65536_rows"] = hipblaslt_gemm(65536, 512, 512)
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, size=(4096, surrogate.N_FEATURES)).astype(np.float32)
+    y = rng.uniform(-1, 1, size=4096).astype(np.float32)
+    trainer.train_step(x, y, 1e-4)
+    t1 = time.perf_counter()
+    for _ in range(50):
+        trainer.train_step(x, y, 1e-4)
+    step_ms = (time.perf_counter() - t1) * 1e3 / 50
+    trainer.close()
+    return gemm, step_ms
+
+
+def gemm_roofline(gemm: dict) -> dict:
+    dom = gemm["forward_hidden_65536_rows"]
+    return {"bound": "mfma", "kernel": "gemm_nt_kernel",
+            "shape": "65536 x 512 x 512 (forward of a hidden layer, bias + ReLU + bf16 out fused)",
+            "achieved": dom["tflops"], "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": dom["tflops"] / MFMA_BF16_PEAK_TFLOPS, "tile": dom["tile"],
+            # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration)
+            "traffic": GEMM_FWD_65536_TRAFFIC_BYTES,
+            "traffic_copied_from": "profiles/gemm_nt_kernel_surrogate.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
+            "algorithmic_bytes": 2.0 * (65536 * 512 + 512 * 512 + 65536 * 512)}
+
+
+def gemm_bench(args) -> dict:
+    """The surrogate's MFMA products alone (no Monte Carlo): what the gemm_nt_kernel profile measures."""
+    import torch  # noqa: F401  (before the library initialises HIP)
+
+    t0 = time.perf_counter()
+    gemm, step_ms = surrogate_products()
+    dom = gemm["forward_hidden_65536_rows"]
+    return {"metric": "surrogate forward product TFLOP/s", "value": dom["tflops"], "unit": "TFLOP/s", "n_gpus": 1,
+            "steps": 1, "warmup": 0, "ms_per_step": (time.perf_counter() - t0) * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)", "data": "synthetic",
+            "config": {"workload": "surrogate 32 -> 512 x 4 -> 1: fused products of a hidden layer at 4096 and "
+                                   "65536 rows, and a training step at batch 4096", "parallelism": "single GPU"},
+            "roofline": gemm_roofline(gemm), "gemm": gemm, "training_step_ms_b4096": step_ms, "cpu_baseline": None}
 
 
 def surrogate_bench(args) -> dict:
     """Warm start of held-out structures from the trained surrogate against the cold start, and the
     surrogate's MFMA GEMM against the bf16 peak."""
     import numpy as np
+    import torch  # noqa: F401  (before the engine initialises HIP: torch's own device setup then succeeds)
     import psim_amd
     from psim_amd import configs, surrogate, train_surrogate as ts
 
@@ -352,29 +445,8 @@ def surrogate_bench(args) -> dict:
     wall = time.perf_counter() - t0
     s = res["summary"]
     row = s["criteria"][str(ts.HEADLINE_TOL)]
-    # the GEMM kernel at the training shapes (batch 4096, hidden 512) and at one held-out forward pass
     rows = sum(p["n_sensors"] for p in params)
-    # the forward reads the transposed weight copy (B stored [N][K]); weight gradients are split over K
-    shapes = {"forward_hidden_b4096": (4096, 512, 512, False, True),
-              "weight_grad_hidden_b4096": (512, 512, 4096, True, False),
-              "input_grad_hidden_b4096": (4096, 512, 512, False, True),
-              f"forward_hidden_{rows}_rows": (rows, 512, 512, False, True),
-              "forward_hidden_65536_rows": (65536, 512, 512, False, True)}
-    gemm = {}
-    for name, (M, N, K, ta, tb) in shapes.items():
-        ms = surrogate.time_gemm(M, N, K, ta, tb, reps=50)
-        gemm[name] = {"M": M, "N": N, "K": K, "ms": ms, "tflops": 2.0 * M * N * K / (ms / 1e3) / 1e12}
-    # one training step (forward, backward, Adam) at batch 4096, timed over 50 steps
-    rng = np.random.default_rng(0)
-    x = rng.uniform(-1, 1, size=(4096, surrogate.N_FEATURES)).astype(np.float32)
-    y = rng.uniform(-1, 1, size=4096).astype(np.float32)
-    trainer = surrogate.Surrogate(seed=1)
-    trainer.train_step(x, y, 1e-4)
-    t1 = time.perf_counter()
-    for _ in range(50):
-        trainer.train_step(x, y, 1e-4)
-    step_ms = (time.perf_counter() - t1) * 1e3 / 50
-    dom = gemm["forward_hidden_65536_rows"]
+    gemm, step_ms = surrogate_products(rows)
     line = {
         "metric": "time-to-steady-state cold/warm", "value": row["cold_over_warm_total"], "unit": "x",
         "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": wall * 1e3, "higher_is_better": True,
@@ -389,9 +461,7 @@ def surrogate_bench(args) -> dict:
                    "parallelism": "single GPU"},
         "cold_over_warm_gpu_seconds": s["gpu_seconds_to_steady_state"]["cold_over_warm"],
         "time_to_steady_state": s,
-        "roofline": {"bound": "mfma", "kernel": "gemm_bf16_kernel", "shape": "65536 x 512 x 512 (forward, hidden)",
-                     "achieved": dom["tflops"], "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": dom["tflops"] / MFMA_BF16_PEAK_TFLOPS, "traffic": None},
+        "roofline": gemm_roofline(gemm),
         "gemm": gemm,
         "training_step_ms_b4096": step_ms,
         "cpu_baseline": None,
@@ -413,7 +483,7 @@ def surrogate_bench(args) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["lshape", "film", "batch", "surrogate"], default="lshape")
+    ap.add_argument("--workload", choices=["lshape", "film", "batch", "surrogate", "gemm"], default="lshape")
     ap.add_argument("--structures", type=int, default=None, help="held-out structures (surrogate workload)")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
@@ -449,6 +519,9 @@ def main() -> int:
         return 0
     if args.workload == "surrogate":
         print(json.dumps(surrogate_bench(args)), flush=True)
+        return 0
+    if args.workload == "gemm":
+        print(json.dumps(gemm_bench(args)), flush=True)
         return 0
 
     import psim_amd

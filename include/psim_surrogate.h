@@ -42,7 +42,7 @@ int psim_mlp_forward(psim_mlp* mlp, const float* x, int32_t rows, float* y);
  * same kernels and launch shapes as psim_mlp_forward, bit-identical results, with every buffer beyond
  * the caller's x and y carved from a 64 MiB device array that the library holds on each GPU it runs
  * on until the process ends.  Takes at most as many rows as that array holds for this network
- * (15616 for 32 -> 512 x 4); more is refused with PSIM_E_INVALID and a message.  Every call on one GPU
+ * (29056 for 32 -> 512 x 4); more is refused with PSIM_E_INVALID and a message.  Every call on one GPU
  * uses that GPU's one array: calls that overlap on one GPU (two threads, or two networks on
  * different streams) race. */
 int psim_mlp_forward_noalloc(psim_mlp* mlp, const float* x, int32_t rows, float* y);
@@ -62,6 +62,17 @@ int psim_mlp_gemm_check(int device, int32_t ta, int32_t tb, int32_t M, int32_t N
  * events around `reps` launches on a stream of its own: the measurement behind its roofline. */
 int psim_mlp_time_gemm(int device, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, int32_t reps,
     float* ms_per_launch);
+
+/* Average duration (ms) of one of the network's own fused products of its first hidden x hidden
+ * layer at `rows` rows, on the network's buffers and stream (HIP events around `reps` launches):
+ *   kind 0  forward: bias, ReLU and bf16 rounding fused, bf16 activations out (inference);
+ *   kind 1  forward in training: the same plus the transposed activation copy;
+ *   kind 2  weight gradient: transposed activations x transposed gradient, split over the rows;
+ *   kind 3  input gradient through the ReLU: bf16 gradient out, row-major and transposed.
+ * *tile (optional) receives the tile the autotuner chose, as 1000 x rows + columns (128128 ...).
+ * Needs at least two hidden layers. */
+int psim_mlp_time_product(psim_mlp* mlp, int32_t kind, int32_t rows, int32_t reps, float* ms_per_launch,
+    int32_t* tile);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,11 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <array>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -231,70 +235,527 @@ void launch_gemm_t(const uint16_t* A, const uint16_t* B, float* C, int M, int N,
     hipLaunchKernelGGL(sum_splits_kernel, dim3(ew_blocks(n)), dim3(EW_BLOCK), 0, st, ws, C, used, n);
 }
 
-void launch_gemm(bool ta, bool tb, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
-    hipStream_t st, float* ws = nullptr, size_t ws_elems = 0) {
-    if (ta && tb) {
+// ---------------------------------------------------------------------------------------------
+// K-major products, C[M][N] = A[M][K] x B[N][K]^T: the layout of every product the network runs
+// (forward: activations x transposed weights; input gradient: gradient x weights; weight gradient:
+// the transposed activation copy x the transposed gradient copy, both written by the kernels that
+// produce them).
+//
+// A block of WGM x WGN waves computes a BM x BN tile, each wave (BM / WGM) x (BN / WGN) on
+// 16 x 16 x 32 MFMA accumulators.  K advances 64 per pipeline step: every thread loads its 16-byte
+// vectors of the A and B tiles two steps ahead into registers while the waves run the current
+// step's MFMAs from LDS (two 32-deep halves), and stores the next step's into the other LDS
+// buffer; one barrier per step.  The epilogue stages the FP32 tile in LDS (over the pipeline buffers) and writes it with
+// 16-byte stores, row-major and, when asked, column-major (the transposed copy a later product
+// reads K-major).
+// Blocks are dealt to the 8 XCDs in turn; tiles are numbered so that XCD x takes the x-th
+// contiguous range of tiles, N tiles fastest, so the tiles of one BM-row band of A share an L2.
+// A product split over K writes partial tiles to a workspace; the last block of a tile to finish
+// (a per-tile arrival counter) sums the partials in split order, so the result does not depend on
+// which block finishes last, and resets the counter to zero for the next product.
+constexpr int KB = 64;  // K per pipeline step
+constexpr int KP = KB + 8;  // LDS row pitch of the operand tiles (bf16): 144 bytes
+constexpr int XCDS = 8;
+constexpr int NT_COUNTERS = 4096;  // arrival counters: tiles of one split product at most
+
+enum : int { EPI_F32 = 0, EPI_FWD = 1, EPI_BACK = 2 };
+
+struct NtArgs {
+    const uint16_t* A;  // [M][K] bf16
+    const uint16_t* B;  // [N][K] bf16
+    int M, N, K;
+    int vec_a, vec_b;  // operand rows may be read as 16-byte vectors (K % 8 == 0, aligned bases)
+    int vec_out, vec_out_t;  // 16-byte stores of result rows (N % 8 == 0) / transposed rows (M % 8 == 0)
+    int k_split;  // K per split, a multiple of KB (K rounded up when the product is not split)
+    float* c;  // EPI_F32: C [M][N]; EPI_FWD: pre-activation z [M][N] or null
+    float* ws;  // split products: partial tiles [splits][M][N]
+    unsigned* counters;  // split products summed in the kernel: one arrival counter per tile, zero
+                         // between products (null: the partials are left for sum_splits_kernel)
+    const float* bias;  // EPI_FWD: [N]
+    int relu;  // EPI_FWD: ReLU after the bias
+    const uint16_t* mask;  // EPI_BACK: the bf16 activation [M][N] whose positive entries pass the gradient
+    uint16_t* out;  // EPI_FWD / EPI_BACK: bf16 result [M][N], or null
+    uint16_t* out_t;  // EPI_FWD / EPI_BACK: bf16 result transposed [N][M], or null
+};
+
+template <int BM, int BN, int WGM, int WGN, int OCC, int EPI, bool VEC>
+__global__ void __launch_bounds__(64 * WGM * WGN, OCC) gemm_nt_kernel(NtArgs p) {
+    constexpr int NT_THREADS = 64 * WGM * WGN;
+    constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / 16, TN = WN / 16;
+    constexpr int AV = BM * KB / 8 / NT_THREADS, BV = BN * KB / 8 / NT_THREADS;
+    constexpr int CP = BN + 4;  // staging pitch (floats): the 4 row groups of an MFMA result fall on 2 bank halves
+    constexpr int LOOP_BYTES = 2 * (BM + BN) * KP * 2;
+    constexpr int STAGE_BYTES = BM * CP * 4;
+    constexpr int SMEM = LOOP_BYTES > STAGE_BYTES ? LOOP_BYTES : STAGE_BYTES;
+    constexpr int CH = BN / 8;  // 8-column chunks of a tile row
+    static_assert(AV >= 1 && BV >= 1 && TM >= 1 && TN >= 1, "tile too small for the thread count");
+    __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
+    __shared__ int last_block;
+    // operand tiles as 16-byte vectors of 8 bf16: [2][rows][KP / 8], the last vector of a row padding
+    auto As = reinterpret_cast<u32x4 (*)[BM][KP / 8]>(smem);
+    auto Bs = reinterpret_cast<u32x4 (*)[BN][KP / 8]>(smem + 2 * BM * KP * 2);
+    auto Cs = reinterpret_cast<float (*)[CP]>(smem);  // [BM][CP], after the main loop
+
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int wm0 = (wave / WGN) * WM, wn0 = (wave % WGN) * WN;
+    const int tiles_n = (p.N + BN - 1) / BN;
+    const int tiles = tiles_n * ((p.M + BM - 1) / BM);
+    int t = blockIdx.x;
+    if (tiles % XCDS == 0) { t = (t % XCDS) * (tiles / XCDS) + t / XCDS; }
+    const int m0 = (t / tiles_n) * BM, n0 = (t % tiles_n) * BN;
+    const int split = blockIdx.y;
+    const int kb = split * p.k_split, ke = min(p.K, kb + p.k_split);
+    const int nk = (ke - kb + KB - 1) / KB;
+
+    // rows r0 .. r0 + R of an operand, K step k0 .. k0 + KB, into registers: vector v covers row
+    // v / 8, K columns (v % 8) * 8 .. + 8; what lies beyond the rows or this split's K loads as zero.
+    // VEC (K % 8 == 0, aligned operands): raw buffer loads over this block's rows of the operand,
+    // whose record count ends at the matrix's last row, so the hardware returns zeros past it; a
+    // vector past this split's K gets an offset beyond the records.  No branch and no use of the
+    // loaded value, so the loads of a step stay in flight until their LDS stores.
+    u32x4 ra[AV], rb[BV];
+    const __amdgpu_buffer_rsrc_t rs_a = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint16_t*>(p.A + (int64_t)m0 * p.K), (short)0, min(BM, p.M - m0) * p.K * 2, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rs_b = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint16_t*>(p.B + (int64_t)n0 * p.K), (short)0, min(BN, p.N - n0) * p.K * 2, 0x00020000);
+    auto load = [&](__amdgpu_buffer_rsrc_t rs, const uint16_t* __restrict__ X, int rows, int r0, int k0, u32x4* reg,
+                    int count) {
+        for (int i = 0; i < count; ++i) {
+            const int v = tid + i * NT_THREADS;
+            const int r = r0 + v / (KB / 8), k = k0 + (v % (KB / 8)) * 8;
+            if (VEC) {
+                const int off = k < ke ? ((v / (KB / 8)) * p.K + k) * 2 : 0x7ffffff0;
+                reg[i] = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+            } else {
+                uint16_t e[8];
+                for (int j = 0; j < 8; ++j) { e[j] = (r < rows && k + j < ke) ? X[(int64_t)r * p.K + k + j] : (uint16_t)0; }
+                std::memcpy(&reg[i], e, 16);
+            }
+        }
+    };
+    auto store = [&](u32x4 (*S)[KP / 8], const u32x4* reg, int count) {
+        for (int i = 0; i < count; ++i) {
+            const int v = tid + i * NT_THREADS;
+            S[v / (KB / 8)][v % (KB / 8)] = reg[i];
+        }
+    };
+
+    f32x4 acc[TM][TN];
+    for (int i = 0; i < TM; ++i) {
+        for (int j = 0; j < TN; ++j) { acc[i][j] = f32x4{ 0.f, 0.f, 0.f, 0.f }; }
+    }
+    const int kq = lane / 16;  // the lane's 8-wide K vector within a 32-deep MFMA step
+    auto compute = [&](int buf) {
+        for (int h = 0; h < KB / 32; ++h) {
+            bf16x8 fa[TM], fb[TN];
+            for (int i = 0; i < TM; ++i) { fa[i] = __builtin_bit_cast(bf16x8, As[buf][wm0 + i * 16 + lane % 16][h * 4 + kq]); }
+            for (int j = 0; j < TN; ++j) { fb[j] = __builtin_bit_cast(bf16x8, Bs[buf][wn0 + j * 16 + lane % 16][h * 4 + kq]); }
+            for (int i = 0; i < TM; ++i) {
+                for (int j = 0; j < TN; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+    };
+    // Two register sets, loaded two steps ahead of the MFMAs: step s is loaded into set s % 2 during
+    // step s - 2 and stored into LDS buffer s % 2 at the end of step s - 1 (the loop is unrolled by
+    // two so the sets are named statically).
+    u32x4 ra1[AV], rb1[BV];
+    load(rs_a, p.A, p.M, m0, kb, ra, AV);
+    load(rs_b, p.B, p.N, n0, kb, rb, BV);
+    if (nk > 1) {
+        load(rs_a, p.A, p.M, m0, kb + KB, ra1, AV);
+        load(rs_b, p.B, p.N, n0, kb + KB, rb1, BV);
+    }
+    store(As[0], ra, AV);
+    store(Bs[0], rb, BV);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt += 2) {
+        if (kt + 2 < nk) {
+            load(rs_a, p.A, p.M, m0, kb + (kt + 2) * KB, ra, AV);
+            load(rs_b, p.B, p.N, n0, kb + (kt + 2) * KB, rb, BV);
+        }
+        compute(0);
+        if (kt + 1 < nk) {
+            store(As[1], ra1, AV);
+            store(Bs[1], rb1, BV);
+        }
+        __syncthreads();
+        if (kt + 1 >= nk) { break; }
+        if (kt + 3 < nk) {
+            load(rs_a, p.A, p.M, m0, kb + (kt + 3) * KB, ra1, AV);
+            load(rs_b, p.B, p.N, n0, kb + (kt + 3) * KB, rb1, BV);
+        }
+        compute(1);
+        if (kt + 2 < nk) {
+            store(As[0], ra, AV);
+            store(Bs[0], rb, BV);
+        }
+        __syncthreads();
+    }
+
+    // D fragment: lane holds column lane % 16, rows 4 (lane / 16) + e
+    for (int i = 0; i < TM; ++i) {
+        for (int j = 0; j < TN; ++j) {
+            for (int e = 0; e < 4; ++e) { Cs[wm0 + i * 16 + (lane / 16) * 4 + e][wn0 + j * 16 + lane % 16] = acc[i][j][e]; }
+        }
+    }
+    __syncthreads();
+
+    // FP32 tile rows to X [M][N] (8-column chunks)
+    auto write_rows = [&](float* __restrict__ X) {
+        for (int c = tid; c < BM * CH; c += NT_THREADS) {
+            const int r = c / CH, cc = (c % CH) * 8, m = m0 + r, n = n0 + cc;
+            if (m >= p.M || n >= p.N) { continue; }
+            float* dst = X + (int64_t)m * p.N + n;
+            if (p.vec_out && n + 8 <= p.N) {
+                std::memcpy(dst, &Cs[r][cc], 16);
+                std::memcpy(dst + 4, &Cs[r][cc + 4], 16);
+            } else {
+                for (int j = 0; j < 8 && n + j < p.N; ++j) { dst[j] = Cs[r][cc + j]; }
+            }
+        }
+    };
+
+    if (EPI == EPI_F32) {
+        if (gridDim.y == 1) {
+            write_rows(p.c);
+            return;
+        }
+        write_rows(p.ws + (int64_t)split * p.M * p.N);
+        if (!p.counters) { return; }  // the partials are summed by sum_splits_kernel
+        // every thread's partial stores have completed at the barrier; one thread then makes them
+        // visible device-wide (release), counts the arrival and, in the last block, acquires the
+        // other blocks' partials for the whole block
+        __syncthreads();
+        if (tid == 0) {
+            __threadfence();
+            last_block = atomicAdd(&p.counters[t], 1u) == gridDim.y - 1;
+            if (last_block) { __threadfence(); }
+        }
+        __syncthreads();
+        if (!last_block) { return; }
+        const int64_t slice = (int64_t)p.M * p.N;
+        for (int c = tid; c < BM * CH; c += NT_THREADS) {
+            const int r = c / CH, cc = (c % CH) * 8, m = m0 + r, n = n0 + cc;
+            if (m >= p.M || n >= p.N) { continue; }
+            const int64_t at = (int64_t)m * p.N + n;
+            if (p.vec_out && n + 8 <= p.N) {
+                f32x4 s0 = f32x4{ 0.f, 0.f, 0.f, 0.f }, s1 = s0;
+                for (unsigned q = 0; q < gridDim.y; ++q) {
+                    f32x4 v0, v1;
+                    std::memcpy(&v0, p.ws + q * slice + at, 16);
+                    std::memcpy(&v1, p.ws + q * slice + at + 4, 16);
+                    s0 += v0;
+                    s1 += v1;
+                }
+                std::memcpy(p.c + at, &s0, 16);
+                std::memcpy(p.c + at + 4, &s1, 16);
+            } else {
+                for (int j = 0; j < 8 && n + j < p.N; ++j) {
+                    float s = 0.f;
+                    for (unsigned q = 0; q < gridDim.y; ++q) { s += p.ws[q * slice + at + j]; }
+                    p.c[at + j] = s;
+                }
+            }
+        }
+        if (tid == 0) { atomicExch(&p.counters[t], 0u); }
+        return;
+    }
+
+    // EPI_FWD: z = acc + bias, a = bf16(relu ? max(z, 0) : z).  EPI_BACK: d = mask > 0 ? acc : 0,
+    // rounded to bf16.  Pass 1 writes rows and leaves z (or d) in the staging tile for pass 2,
+    // which writes the transposed copy from its columns.
+    for (int c = tid; c < BM * CH; c += NT_THREADS) {
+        const int r = c / CH, cc = (c % CH) * 8, m = m0 + r, n = n0 + cc;
+        if (m >= p.M || n >= p.N) { continue; }
+        const int64_t at = (int64_t)m * p.N + n;
+        float v[8];
+        std::memcpy(v, &Cs[r][cc], 16);
+        std::memcpy(v + 4, &Cs[r][cc + 4], 16);
+        uint16_t o[8];
+        if (p.vec_out && n + 8 <= p.N) {  // whole chunk: vector loads and stores
+            if (EPI == EPI_FWD) {
+                float b[8];
+                std::memcpy(b, p.bias + n, 16);
+                std::memcpy(b + 4, p.bias + n + 4, 16);
+                for (int j = 0; j < 8; ++j) {
+                    v[j] += b[j];
+                    o[j] = f32_to_bf16(p.relu ? fmaxf(v[j], 0.f) : v[j]);
+                }
+                if (p.c) {
+                    std::memcpy(p.c + at, v, 16);
+                    std::memcpy(p.c + at + 4, v + 4, 16);
+                }
+            } else {
+                uint16_t mk[8];
+                std::memcpy(mk, p.mask + at, 16);
+                for (int j = 0; j < 8; ++j) {
+                    v[j] = bf16_to_f32(mk[j]) > 0.f ? v[j] : 0.f;
+                    o[j] = f32_to_bf16(v[j]);
+                }
+            }
+            if (p.out) { std::memcpy(p.out + at, o, 16); }
+        } else {
+            const int nn = min(8, p.N - n);
+            for (int j = 0; j < 8; ++j) {
+                if (EPI == EPI_FWD) {
+                    v[j] = j < nn ? v[j] + p.bias[n + j] : 0.f;
+                    o[j] = f32_to_bf16(p.relu ? fmaxf(v[j], 0.f) : v[j]);
+                } else {
+                    v[j] = (j < nn && bf16_to_f32(p.mask[at + j]) > 0.f) ? v[j] : 0.f;
+                    o[j] = f32_to_bf16(v[j]);
+                }
+            }
+            for (int j = 0; j < nn; ++j) {
+                if (EPI == EPI_FWD && p.c) { p.c[at + j] = v[j]; }
+                if (p.out) { p.out[at + j] = o[j]; }
+            }
+        }
+        if (p.out_t) {
+            std::memcpy(&Cs[r][cc], v, 16);
+            std::memcpy(&Cs[r][cc + 4], v + 4, 16);
+        }
+    }
+    if (!p.out_t) { return; }
+    __syncthreads();
+    constexpr int RCH = BM / 8;  // 8-row chunks of a tile column
+    for (int c = tid; c < BN * RCH; c += NT_THREADS) {
+        const int col = c % BN, r8 = (c / BN) * 8, n = n0 + col, m = m0 + r8;
+        if (n >= p.N || m >= p.M) { continue; }
+        uint16_t o[8];
+        for (int j = 0; j < 8; ++j) {
+            const float x = Cs[r8 + j][col];
+            o[j] = f32_to_bf16(EPI == EPI_FWD && p.relu ? fmaxf(x, 0.f) : x);
+        }
+        uint16_t* dst = p.out_t + (int64_t)n * p.M + m;
+        if (p.vec_out_t && m + 8 <= p.M) {
+            std::memcpy(dst, o, 16);
+        } else {
+            for (int j = 0; j < 8 && m + j < p.M; ++j) { dst[j] = o[j]; }
+        }
+    }
+}
+
+// Tile shapes of the K-major kernel (block tile and its waves); which one runs a product of a given
+// shape is timed once (nt_choose).  The kernel's register budget follows the waves per SIMD its
+// LDS allows (4 for the 128 x 128 and 64 x 64 tiles, 2 for the others).  The tile does not change
+// a result: every output element sums its K range in the same order whatever the tile.
+struct NtTile {
+    int bm, bn, wgm, wgn;
+};
+constexpr NtTile NT_TILES[] = { { 256, 128, 4, 2 }, { 128, 128, 2, 4 }, { 128, 64, 2, 2 }, { 64, 128, 2, 2 },
+    { 64, 64, 2, 2 } };
+constexpr int NT_NUM_TILES = 5;
+
+int nt_tiles_of(int tile, int M, int N) {
+    return ((M + NT_TILES[tile].bm - 1) / NT_TILES[tile].bm) * ((N + NT_TILES[tile].bn - 1) / NT_TILES[tile].bn);
+}
+
+// Splits over K of a K-major product: fixed by its shape alone (never by timing), so a product's
+// summation order, and its result, is the same in every run.  Products with few 64 x 64 output
+// tiles and a long K (the weight gradients, K = batch rows) are split into parts of at least
+// SPLIT_MIN_K.
+int nt_splits(int M, int N, int K) {
+    const int64_t tiles = (int64_t)((M + 63) / 64) * ((N + 63) / 64);
+    if (tiles >= SPLIT_TARGET_BLOCKS || K < 2 * SPLIT_MIN_K) { return 1; }
+    int s = (int)std::min<int64_t>(MAX_SPLITS, (SPLIT_TARGET_BLOCKS + tiles - 1) / tiles);
+    return std::max(1, std::min(s, K / SPLIT_MIN_K));
+}
+
+template <int EPI>
+void launch_nt_tile(int tile, const NtArgs& a, int splits, hipStream_t st) {
+    const dim3 grid((unsigned)nt_tiles_of(tile, a.M, a.N), (unsigned)splits);
+    const NtTile& t = NT_TILES[tile];
+    const dim3 block((unsigned)(64 * t.wgm * t.wgn));
+    if (!(a.vec_a && a.vec_b)) {  // element loads (ragged K or unaligned operands): one tile shape
+        hipLaunchKernelGGL((gemm_nt_kernel<64, 64, 2, 2, 2, EPI, false>), grid, block, 0, st, a);
+        return;
+    }
+    switch (tile) {
+    case 0: hipLaunchKernelGGL((gemm_nt_kernel<256, 128, 4, 2, 2, EPI, true>), grid, block, 0, st, a); break;
+    case 1: hipLaunchKernelGGL((gemm_nt_kernel<128, 128, 2, 4, 4, EPI, true>), grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((gemm_nt_kernel<128, 64, 2, 2, 2, EPI, true>), grid, block, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((gemm_nt_kernel<64, 128, 2, 2, 2, EPI, true>), grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL((gemm_nt_kernel<64, 64, 2, 2, 4, EPI, true>), grid, block, 0, st, a); break;
+    }
+    if (EPI == EPI_F32 && splits > 1 && !a.counters) {
+        const int64_t n = (int64_t)a.M * a.N;
+        hipLaunchKernelGGL(sum_splits_kernel, dim3(ew_blocks(n)), dim3(EW_BLOCK), 0, st, a.ws, a.c, splits, n);
+    }
+}
+
+// How a K-major product runs: a tile shape (NT_TILES index) and, for a product split over K,
+// where its partials are summed: in the kernel by each tile's last block (choice < NT_NUM_TILES),
+// or by sum_splits_kernel after it (choice - NT_NUM_TILES).  Both sum the partials in split order,
+// so every choice gives the same result.
+struct NtChoice {
+    int tile;
+    bool summed_in_kernel;
+};
+
+NtChoice nt_decode(int choice) { return { choice % NT_NUM_TILES, choice < NT_NUM_TILES }; }
+
+template <int EPI>
+void launch_nt_choice(int choice, NtArgs a, int splits, hipStream_t st) {
+    const NtChoice c = nt_decode(choice);
+    if (!c.summed_in_kernel) { a.counters = nullptr; }
+    launch_nt_tile<EPI>(c.tile, a, splits, st);
+}
+
+// Choice before any timing: the largest tile whose blocks fill the chip twice over (2 per CU), else
+// the smallest; split products summed in the kernel.
+int nt_default_choice(int M, int N, int splits) {
+    for (int tile = 0; tile < NT_NUM_TILES; ++tile) {
+        if ((int64_t)nt_tiles_of(tile, M, N) * splits >= 512) { return tile; }
+    }
+    return NT_NUM_TILES - 1;
+}
+
+// Autotuner: the first product of each (device, epilogue, outputs, M, N, K, splits) times every
+// choice (one launch to warm, then HIP events around three) on the caller's stream with the
+// product's own arguments, and keeps the fastest for the rest of the process.  Off while the stream
+// is being captured into a graph and for callers that must not create events (may_tune false):
+// those take a choice already tuned, else the default.
+std::mutex nt_tune_mutex;
+std::map<std::array<int64_t, 7>, int> nt_tuned;
+
+template <int EPI>
+int nt_choose(int device, const NtArgs& a, int splits, hipStream_t st, bool may_tune) {
+    const int64_t outputs = (a.c ? 1 : 0) | (a.out ? 2 : 0) | (a.out_t ? 4 : 0);
+    const std::array<int64_t, 7> key = { device, EPI, outputs, a.M, a.N, a.K, splits };
+    if (!(a.vec_a && a.vec_b)) { return NT_NUM_TILES - 1; }  // element loads: one tile shape
+    // PSIM_GEMM_CHOICE forces a choice (tests run every choice and compare the results)
+    if (const char* forced = std::getenv("PSIM_GEMM_CHOICE")) {
+        const int c = std::atoi(forced);
+        if (c >= 0 && c < (splits > 1 ? 2 * NT_NUM_TILES : NT_NUM_TILES)) { return c; }
+    }
+    std::lock_guard<std::mutex> lock(nt_tune_mutex);
+    const auto it = nt_tuned.find(key);
+    if (it != nt_tuned.end()) { return it->second; }
+    const int fallback = nt_default_choice(a.M, a.N, splits);
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (!may_tune || hipStreamIsCapturing(st, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) {
+        return fallback;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) { (void)hipEventDestroy(e0); }
+        return fallback;
+    }
+    const int choices = splits > 1 ? 2 * NT_NUM_TILES : NT_NUM_TILES;
+    int best = fallback;
+    float best_ms = 0.f;
+    bool ok = true;
+    for (int choice = 0; choice < choices && ok; ++choice) {
+        launch_nt_choice<EPI>(choice, a, splits, st);
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < 3; ++r) { launch_nt_choice<EPI>(choice, a, splits, st); }
+        (void)hipEventRecord(e1, st);
+        float ms = 0.f;
+        ok = hipGetLastError() == hipSuccess && hipEventSynchronize(e1) == hipSuccess
+             && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+        if (ok && (choice == 0 || ms < best_ms)) {
+            best = choice;
+            best_ms = ms;
+        }
+        if (ok && std::getenv("PSIM_TUNE_LOG")) {
+            const NtChoice c = nt_decode(choice);
+            std::fprintf(stderr, "psim tune: epilogue %d outputs %d %d x %d x %d, %d splits%s, tile %d x %d (%d waves): %.2f us\n",
+                EPI, (int)outputs, a.M, a.N, a.K, splits, splits > 1 ? (c.summed_in_kernel ? " summed in kernel" : " summed after") : "",
+                NT_TILES[c.tile].bm, NT_TILES[c.tile].bn, NT_TILES[c.tile].wgm * NT_TILES[c.tile].wgn, ms * 1e3f / 3.f);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ok) { nt_tuned[key] = best; }
+    return ok ? best : fallback;
+}
+
+bool aligned16(const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; }
+
+// Fills the shape-derived fields of a K-major product and launches it as tuned.  Split products
+// need ws (splits x M x N floats) and counters (NT_COUNTERS, zero); without them, or when the tiles
+// outnumber the counters, the product runs unsplit.  Returns the tile index used.
+template <int EPI>
+int launch_nt(int device, NtArgs a, hipStream_t st, bool may_tune, size_t ws_elems) {
+    a.vec_a = aligned16(a.A) && a.K % 8 == 0;
+    a.vec_b = aligned16(a.B) && a.K % 8 == 0;
+    a.vec_out = a.N % 8 == 0 && aligned16(a.c) && aligned16(a.ws) && aligned16(a.out) && aligned16(a.mask);
+    a.vec_out_t = a.M % 8 == 0 && aligned16(a.out_t);
+    int splits = EPI == EPI_F32 && a.ws && a.counters ? nt_splits(a.M, a.N, a.K) : 1;
+    int k_split = (a.K + KB - 1) / KB * KB;
+    if (splits > 1) {
+        k_split = ((a.K + splits - 1) / splits + KB - 1) / KB * KB;
+        splits = (a.K + k_split - 1) / k_split;
+        if ((size_t)splits * (size_t)a.M * (size_t)a.N > ws_elems || nt_tiles_of(NT_NUM_TILES - 1, a.M, a.N) > NT_COUNTERS) {
+            splits = 1;
+            k_split = (a.K + KB - 1) / KB * KB;
+        }
+    }
+    a.k_split = k_split;
+    const int choice = nt_choose<EPI>(device, a, splits, st, may_tune);
+    launch_nt_choice<EPI>(choice, a, splits, st);
+    return nt_decode(choice).tile;
+}
+
+// C = op(A) x op(B) in FP32 (operands as stored: A [K][M] when ta, B [N][K] when tb).  The
+// K-major layout (A [M][K], B [N][K]) runs on the network's kernel, split over K when ws and
+// counters are given and the shape asks for it; the others on gemm_bf16_kernel.
+void launch_gemm(int device, bool ta, bool tb, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
+    hipStream_t st, float* ws, size_t ws_elems, unsigned* counters, bool may_tune) {
+    if (!ta && tb) {
+        NtArgs a{};
+        a.A = A;
+        a.B = B;
+        a.M = M;
+        a.N = N;
+        a.K = K;
+        a.c = C;
+        a.ws = ws;
+        a.counters = counters;
+        launch_nt<EPI_F32>(device, a, st, may_tune, ws_elems);
+    } else if (ta && tb) {
         launch_gemm_t<true, true>(A, B, C, M, N, K, st, ws, ws_elems);
     } else if (ta) {
         launch_gemm_t<true, false>(A, B, C, M, N, K, st, ws, ws_elems);
-    } else if (tb) {
-        launch_gemm_t<false, true>(A, B, C, M, N, K, st, ws, ws_elems);
     } else {
         launch_gemm_t<false, false>(A, B, C, M, N, K, st, ws, ws_elems);
     }
 }
 
-// dst[n][k] = src[k][n] for a K x N bf16 matrix, through a 32 x 32 LDS tile.
-__global__ void transpose_bf16_kernel(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst, int K, int N) {
-    __shared__ uint16_t tile[32][33];
-    const int n0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
-    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 256 threads: 8 rows per pass
-    for (int r = ty; r < 32; r += 8) {
-        const int k = k0 + r, n = n0 + tx;
-        tile[r][tx] = (k < K && n < N) ? src[(int64_t)k * N + n] : (uint16_t)0;
-    }
-    __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
-        const int n = n0 + r, k = k0 + tx;
-        if (n < N && k < K) { dst[(int64_t)n * K + k] = tile[tx][r]; }
-    }
-}
-
-// z += bias (per column); a = relu ? max(z, 0) : z, rounded to bf16.
-__global__ void bias_act_kernel(float* __restrict__ z, const float* __restrict__ bias, uint16_t* __restrict__ a,
-    int64_t rows, int cols, int relu) {
+// a = bf16(x) [rows][n] and, when a_t is given, its transposed copy a_t [n][rows].
+__global__ void input_kernel(const float* __restrict__ x, uint16_t* __restrict__ a, uint16_t* __restrict__ a_t,
+    int64_t rows, int n) {
     const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
-    if (i >= rows * cols) { return; }
-    const float v = z[i] + bias[i % cols];
-    z[i] = v;
-    a[i] = f32_to_bf16(relu ? fmaxf(v, 0.f) : v);
+    if (i >= rows * n) { return; }
+    const uint16_t v = f32_to_bf16(x[i]);
+    a[i] = v;
+    if (a_t) { a_t[(i % n) * rows + i / n] = v; }
 }
 
 // Output-layer gradient of the mean squared error on column 0: dz[r][0] = 2 (y - t) / rows, other
-// columns 0 (bf16), and the loss terms (y - t)^2 / rows per row.
+// columns 0 (bf16, row-major and transposed), and the loss terms (y - t)^2 / rows per row.
 __global__ void loss_grad_kernel(const float* __restrict__ z, const float* __restrict__ t, uint16_t* __restrict__ dz,
-    float* __restrict__ loss_terms, int rows, int cols) {
+    uint16_t* __restrict__ dz_t, float* __restrict__ loss_terms, int rows, int cols) {
     const int r = blockIdx.x * EW_BLOCK + threadIdx.x;
     if (r >= rows) { return; }
     const float d = z[(int64_t)r * cols] - t[r];
     loss_terms[r] = d * d / (float)rows;
-    for (int c = 0; c < cols; ++c) { dz[(int64_t)r * cols + c] = f32_to_bf16(c == 0 ? 2.f * d / (float)rows : 0.f); }
+    for (int c = 0; c < cols; ++c) {
+        const uint16_t v = f32_to_bf16(c == 0 ? 2.f * d / (float)rows : 0.f);
+        dz[(int64_t)r * cols + c] = v;
+        dz_t[(int64_t)c * rows + r] = v;
+    }
 }
 
-// dz = da * (z > 0), rounded to bf16 (da FP32 from the GEMM).
-__global__ void relu_back_kernel(const float* __restrict__ da, const float* __restrict__ z, uint16_t* __restrict__ dz,
-    int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
-    if (i >= n) { return; }
-    dz[i] = f32_to_bf16(z[i] > 0.f ? da[i] : 0.f);
-}
-
-// db[c] = sum over rows of dz[r][c] (one block per column, FP32).
-__global__ void bias_grad_kernel(const uint16_t* __restrict__ dz, float* __restrict__ db, int rows, int cols) {
+// db[c] = sum over rows of dz_t[c][r] (one block per column reading its contiguous row of the
+// transposed gradient; FP32, thread-strided partial sums then a fixed tree).
+__global__ void bias_grad_kernel(const uint16_t* __restrict__ dz_t, float* __restrict__ db, int rows) {
     __shared__ float part[EW_BLOCK];
     const int c = blockIdx.x;
     float s = 0.f;
-    for (int r = threadIdx.x; r < rows; r += EW_BLOCK) { s += bf16_to_f32(dz[(int64_t)r * cols + c]); }
+    for (int r = threadIdx.x; r < rows; r += EW_BLOCK) { s += bf16_to_f32(dz_t[(int64_t)c * rows + r]); }
     part[threadIdx.x] = s;
     __syncthreads();
     for (int off = EW_BLOCK / 2; off > 0; off >>= 1) {
@@ -304,10 +765,31 @@ __global__ void bias_grad_kernel(const uint16_t* __restrict__ dz, float* __restr
     if (threadIdx.x == 0) { db[c] = part[0]; }
 }
 
-// Adam on the FP32 master parameters, then their bf16 copy (weights only; masked entries stay 0).
+// Where each weight matrix lies in the flat parameter vector, for kernels that write its bf16
+// copies: [in][out] at w_off in pb, [out][in] at w_off in pbT.
+constexpr int MAX_WEIGHTS = 17;
+struct LayerTable {
+    int count;
+    int64_t w_off[MAX_WEIGHTS];
+    int in[MAX_WEIGHTS], out[MAX_WEIGHTS];
+};
+
+__device__ inline void store_bf16_copies(const LayerTable& lt, int64_t i, uint16_t b, uint16_t* __restrict__ pb,
+    uint16_t* __restrict__ pbT) {
+    pb[i] = b;
+    for (int l = 0; l < lt.count; ++l) {
+        const int64_t o = i - lt.w_off[l];
+        if (o >= 0 && o < (int64_t)lt.in[l] * lt.out[l]) {
+            pbT[lt.w_off[l] + (o % lt.out[l]) * lt.in[l] + o / lt.out[l]] = b;
+            return;
+        }
+    }
+}
+
+// Adam on the FP32 master parameters, then their bf16 copies (masked entries stay 0).
 __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
-    float* __restrict__ m2, const float* __restrict__ mask, uint16_t* __restrict__ pb, int64_t n, float lr, float b1,
-    float b2, float eps, float c1, float c2) {
+    float* __restrict__ m2, const float* __restrict__ mask, uint16_t* __restrict__ pb, uint16_t* __restrict__ pbT,
+    LayerTable lt, int64_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
     const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) { return; }
     const float gi = g[i] * (mask ? mask[i] : 1.f);
@@ -317,7 +799,14 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, 
     m2[i] = b;
     const float v = p[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
     p[i] = v;
-    if (pb) { pb[i] = f32_to_bf16(v); }
+    store_bf16_copies(lt, i, f32_to_bf16(v), pb, pbT);
+}
+
+// bf16 copies of the FP32 parameters.
+__global__ void params_bf16_kernel(const float* __restrict__ p, uint16_t* __restrict__ pb, uint16_t* __restrict__ pbT,
+    LayerTable lt, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
+    if (i < n) { store_bf16_copies(lt, i, f32_to_bf16(p[i]), pb, pbT); }
 }
 
 __global__ void to_bf16_kernel(const float* __restrict__ x, uint16_t* __restrict__ y, int64_t n) {
@@ -355,17 +844,20 @@ struct psim_mlp {
     uint16_t* pbT = nullptr;  // bf16 weights transposed, [out][in] per layer (K-major for the forward)
     float* ws = nullptr;  // partial products of weight gradients split over K
     size_t ws_elems = 0;
+    unsigned* counters = nullptr;  // per-tile arrival counters of split products (zero between products)
     int64_t step = 0;
-    // batch buffers
+    // batch buffers (cap rows)
     int cap = 0;
-    std::vector<float*> z;  // pre-activations per layer [cap][dims[l + 1]]
-    std::vector<uint16_t*> a;  // activations: a[0] = input, a[l + 1] = output of layer l
-    float* da = nullptr;  // FP32 gradient from a GEMM [cap][hidden]
-    uint16_t* dz = nullptr;  // bf16 gradient of the current layer's pre-activation
-    uint16_t* dz2 = nullptr;
+    std::vector<uint16_t*> a;  // bf16 input of every layer: a[0] = the network's input, a[l] = output of layer l - 1
+    float* zout = nullptr;  // output layer's FP32 pre-activation [cap][PAD_OUT]
     float* xin = nullptr;
     float* t = nullptr;
     float* loss = nullptr;
+    // training buffers (train_cap rows): transposed copies, [features][rows]
+    int train_cap = 0;
+    std::vector<uint16_t*> aT;  // aT[l]: a[l] transposed
+    uint16_t *dz = nullptr, *dz2 = nullptr;  // bf16 gradient of the current layer's pre-activation [rows][features]
+    uint16_t *dzT = nullptr, *dz2T = nullptr;  // the same transposed
     // device address of this GPU's `scratch` array (psim_mlp_forward_noalloc), looked up at create
     unsigned char* scratch = nullptr;
 };
@@ -390,19 +882,32 @@ T* mlp_alloc(size_t n) {
     return p;
 }
 
-void free_batch(psim_mlp* h) {
-    for (float* q : h->z) { (void)hipFree(q); }
-    for (uint16_t* q : h->a) { (void)hipFree(q); }
-    h->z.clear();
-    h->a.clear();
-    void* rest[] = { h->da, h->dz, h->dz2, h->xin, h->t, h->loss };
-    for (void* q : rest) {
+void free_train(psim_mlp* h) {
+    for (uint16_t* q : h->aT) { (void)hipFree(q); }
+    h->aT.clear();
+    uint16_t* rest[] = { h->dz, h->dz2, h->dzT, h->dz2T };
+    for (uint16_t* q : rest) {
         if (q) { (void)hipFree(q); }
     }
-    h->da = nullptr;
-    h->dz = h->dz2 = nullptr;
-    h->xin = h->t = h->loss = nullptr;
+    h->dz = h->dz2 = h->dzT = h->dz2T = nullptr;
+    h->train_cap = 0;
+}
+
+void free_batch(psim_mlp* h) {
+    for (uint16_t* q : h->a) { (void)hipFree(q); }
+    h->a.clear();
+    float* rest[] = { h->zout, h->xin, h->t, h->loss };
+    for (float* q : rest) {
+        if (q) { (void)hipFree(q); }
+    }
+    h->zout = h->xin = h->t = h->loss = nullptr;
     h->cap = 0;
+}
+
+int widest_dim(const psim_mlp* h) {
+    int widest = 0;
+    for (int d : h->dims) { widest = std::max(widest, d); }
+    return widest;
 }
 
 int ensure_batch(psim_mlp* h, int rows) {
@@ -410,18 +915,12 @@ int ensure_batch(psim_mlp* h, int rows) {
     free_batch(h);
     const int cap = ((rows + BATCH_ROUND - 1) / BATCH_ROUND) * BATCH_ROUND;
     const int L = (int)h->dims.size() - 1;
-    for (int l = 0; l < L; ++l) { h->z.push_back(mlp_alloc<float>((size_t)cap * h->dims[l + 1])); }
-    for (int l = 0; l <= L; ++l) { h->a.push_back(mlp_alloc<uint16_t>((size_t)cap * h->dims[l])); }
-    int widest = 0;
-    for (int d : h->dims) { widest = std::max(widest, d); }
-    h->da = mlp_alloc<float>((size_t)cap * widest);
-    h->dz = mlp_alloc<uint16_t>((size_t)cap * widest);
-    h->dz2 = mlp_alloc<uint16_t>((size_t)cap * widest);
+    for (int l = 0; l < L; ++l) { h->a.push_back(mlp_alloc<uint16_t>((size_t)cap * h->dims[l])); }
+    h->zout = mlp_alloc<float>((size_t)cap * PAD_OUT);
     h->xin = mlp_alloc<float>((size_t)cap * h->n_in);
     h->t = mlp_alloc<float>((size_t)cap);
     h->loss = mlp_alloc<float>((size_t)cap);
-    bool ok = h->da && h->dz && h->dz2 && h->xin && h->t && h->loss;
-    for (float* q : h->z) { ok = ok && q; }
+    bool ok = h->zout && h->xin && h->t && h->loss;
     for (uint16_t* q : h->a) { ok = ok && q; }
     if (!ok) {
         free_batch(h);
@@ -431,73 +930,120 @@ int ensure_batch(psim_mlp* h, int rows) {
     return PSIM_OK;
 }
 
-template <bool TA, bool TB>
-int gemm(psim_mlp* h, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K, bool split = false) {
-    launch_gemm(TA, TB, A, B, C, M, N, K, h->stream, split ? h->ws : nullptr, split ? h->ws_elems : 0);
-    MLP_CHECK(h, hipGetLastError());
-    return PSIM_OK;
-}
-
-// pbT from pb: every layer's weights transposed.
-int refresh_transposed(psim_mlp* h) {
-    for (size_t l = 0; l + 1 < h->dims.size(); ++l) {
-        const int K = h->dims[l], N = h->dims[l + 1];
-        const dim3 grid((unsigned)((N + 31) / 32), (unsigned)((K + 31) / 32));
-        hipLaunchKernelGGL(transpose_bf16_kernel, grid, dim3(256), 0, h->stream, h->pb + h->w_off[l],
-            h->pbT + h->w_off[l], K, N);
-        MLP_CHECK(h, hipGetLastError());
-    }
-    return PSIM_OK;
-}
-
-// Forward pass over `rows` inputs already in h->xin; leaves z and a of every layer.
-int forward(psim_mlp* h, int rows) {
+int ensure_train(psim_mlp* h, int rows) {
     int rc;
+    if ((rc = ensure_batch(h, rows)) != 0) { return rc; }
+    if (rows <= h->train_cap) { return PSIM_OK; }
+    free_train(h);
+    const int cap = ((rows + BATCH_ROUND - 1) / BATCH_ROUND) * BATCH_ROUND;
     const int L = (int)h->dims.size() - 1;
-    hipLaunchKernelGGL(to_bf16_kernel, dim3(ew_blocks((int64_t)rows * h->n_in)), dim3(EW_BLOCK), 0, h->stream, h->xin,
-        h->a[0], (int64_t)rows * h->n_in);
+    const size_t wide = (size_t)cap * widest_dim(h);
+    for (int l = 0; l < L; ++l) { h->aT.push_back(mlp_alloc<uint16_t>((size_t)cap * h->dims[l])); }
+    h->dz = mlp_alloc<uint16_t>(wide);
+    h->dz2 = mlp_alloc<uint16_t>(wide);
+    h->dzT = mlp_alloc<uint16_t>(wide);
+    h->dz2T = mlp_alloc<uint16_t>(wide);
+    bool ok = h->dz && h->dz2 && h->dzT && h->dz2T;
+    for (uint16_t* q : h->aT) { ok = ok && q; }
+    if (!ok) {
+        free_train(h);
+        return mlp_fail(h, PSIM_E_DEVICE, "hipMalloc failed for the surrogate training buffers");
+    }
+    h->train_cap = cap;
+    return PSIM_OK;
+}
+
+LayerTable layer_table(const psim_mlp* h) {
+    LayerTable lt{};
+    lt.count = (int)h->dims.size() - 1;
+    for (int l = 0; l < lt.count; ++l) {
+        lt.w_off[l] = h->w_off[l];
+        lt.in[l] = h->dims[l];
+        lt.out[l] = h->dims[l + 1];
+    }
+    return lt;
+}
+
+// Forward pass over `rows` inputs at xin: layer l reads a[l] (a[0] is written here from xin) and
+// writes its bf16 activation into a[l + 1]; the output layer writes its FP32 pre-activation to zout.
+// With aT (training) every layer's input is also written transposed, aT[l], for the weight gradients.
+int forward_pass(psim_mlp* h, int rows, const float* xin, uint16_t* const* a, uint16_t* const* aT, float* zout,
+    bool may_tune) {
+    const int L = (int)h->dims.size() - 1;
+    hipLaunchKernelGGL(input_kernel, dim3(ew_blocks((int64_t)rows * h->n_in)), dim3(EW_BLOCK), 0, h->stream, xin, a[0],
+        aT ? aT[0] : nullptr, (int64_t)rows, h->n_in);
     MLP_CHECK(h, hipGetLastError());
     for (int l = 0; l < L; ++l) {
-        const int K = h->dims[l], N = h->dims[l + 1];
-        // B = W^T stored [N][K]: both operands K-major
-        if ((rc = gemm<false, true>(h, h->a[l], h->pbT + h->w_off[l], h->z[l], rows, N, K)) != 0) { return rc; }
-        hipLaunchKernelGGL(bias_act_kernel, dim3(ew_blocks((int64_t)rows * N)), dim3(EW_BLOCK), 0, h->stream, h->z[l],
-            h->p + h->b_off[l], h->a[l + 1], (int64_t)rows, N, l + 1 < L ? 1 : 0);
+        const bool last = l + 1 == L;
+        NtArgs f{};
+        f.A = a[l];
+        f.B = h->pbT + h->w_off[l];  // W^T stored [out][in]: both operands K-major
+        f.M = rows;
+        f.N = h->dims[l + 1];
+        f.K = h->dims[l];
+        f.c = last ? zout : nullptr;
+        f.bias = h->p + h->b_off[l];
+        f.relu = last ? 0 : 1;
+        f.out = last ? nullptr : a[l + 1];
+        f.out_t = (aT && !last) ? aT[l + 1] : nullptr;
+        launch_nt<EPI_FWD>(h->device, f, h->stream, may_tune, 0);
         MLP_CHECK(h, hipGetLastError());
     }
     return PSIM_OK;
 }
 
-// Loss and gradients of every parameter (into h->g) for targets in h->t.
+int forward(psim_mlp* h, int rows, bool train) {
+    return forward_pass(h, rows, h->xin, h->a.data(), train ? h->aT.data() : nullptr, h->zout, true);
+}
+
+// Loss and gradients of every parameter (into h->g) for targets in h->t, after forward(train).
+// Layer l: dW_l = a_l^T dz_l on the transposed copies (split over the batch rows), db_l = row sums of
+// dz_l^T, and dz_{l-1} = (dz_l W_l^T) where a_l > 0: the ReLU of layer l - 1 passed a positive
+// pre-activation exactly where its bf16 output a_l is positive (a pre-activation below 2^-134
+// would round to a zero activation and drop its gradient).
 int backward(psim_mlp* h, int rows) {
-    int rc;
     const int L = (int)h->dims.size() - 1;
-    hipLaunchKernelGGL(loss_grad_kernel, dim3(ew_blocks(rows)), dim3(EW_BLOCK), 0, h->stream, h->z[L - 1], h->t, h->dz,
-        h->loss, rows, h->dims[L]);
+    hipLaunchKernelGGL(loss_grad_kernel, dim3(ew_blocks(rows)), dim3(EW_BLOCK), 0, h->stream, h->zout, h->t, h->dz,
+        h->dzT, h->loss, rows, h->dims[L]);
     MLP_CHECK(h, hipGetLastError());
-    uint16_t* dz = h->dz;
-    uint16_t* spare = h->dz2;
+    uint16_t *dz = h->dz, *dzT = h->dzT, *sp = h->dz2, *spT = h->dz2T;
     for (int l = L - 1; l >= 0; --l) {
         const int K = h->dims[l], N = h->dims[l + 1];
-        // dW[K][N] = a_l^T [K][rows] x dz [rows][N]; db[N] = column sums of dz
-        if ((rc = gemm<true, false>(h, h->a[l], dz, h->g + h->w_off[l], K, N, rows, true)) != 0) { return rc; }
-        hipLaunchKernelGGL(bias_grad_kernel, dim3((unsigned)N), dim3(EW_BLOCK), 0, h->stream, dz, h->g + h->b_off[l],
-            rows, N);
+        NtArgs w{};
+        w.A = h->aT[l];
+        w.B = dzT;
+        w.M = K;
+        w.N = N;
+        w.K = rows;
+        w.c = h->g + h->w_off[l];
+        w.ws = h->ws;
+        w.counters = h->counters;
+        launch_nt<EPI_F32>(h->device, w, h->stream, true, h->ws_elems);
+        MLP_CHECK(h, hipGetLastError());
+        hipLaunchKernelGGL(bias_grad_kernel, dim3((unsigned)N), dim3(EW_BLOCK), 0, h->stream, dzT, h->g + h->b_off[l],
+            rows);
         MLP_CHECK(h, hipGetLastError());
         if (l == 0) { break; }
-        // da[rows][K] = dz [rows][N] x W^T [N][K] (W stored [K][N]), then through the ReLU of layer l - 1
-        if ((rc = gemm<false, true>(h, dz, h->pb + h->w_off[l], h->da, rows, K, N)) != 0) { return rc; }
-        hipLaunchKernelGGL(relu_back_kernel, dim3(ew_blocks((int64_t)rows * K)), dim3(EW_BLOCK), 0, h->stream, h->da,
-            h->z[l - 1], spare, (int64_t)rows * K);
+        NtArgs d{};
+        d.A = dz;
+        d.B = h->pb + h->w_off[l];  // W stored [in][out]: K-major for this product
+        d.M = rows;
+        d.N = K;
+        d.K = N;
+        d.mask = h->a[l];
+        d.out = l > 1 ? sp : nullptr;  // layer 0's gradient is read only transposed
+        d.out_t = spT;
+        launch_nt<EPI_BACK>(h->device, d, h->stream, true, 0);
         MLP_CHECK(h, hipGetLastError());
-        std::swap(dz, spare);
+        std::swap(dz, sp);
+        std::swap(dzT, spT);
     }
     return PSIM_OK;
 }
 
 int load_batch(psim_mlp* h, const float* x, const float* t, int rows) {
     int rc;
-    if ((rc = ensure_batch(h, rows)) != 0) { return rc; }
+    if ((rc = t ? ensure_train(h, rows) : ensure_batch(h, rows)) != 0) { return rc; }
     MLP_CHECK(h, hipMemcpyAsync(h->xin, x, sizeof(float) * (size_t)rows * h->n_in, hipMemcpyHostToDevice, h->stream));
     if (t) { MLP_CHECK(h, hipMemcpyAsync(h->t, t, sizeof(float) * (size_t)rows, hipMemcpyHostToDevice, h->stream)); }
     return PSIM_OK;
@@ -526,10 +1072,10 @@ int noalloc_scratch(psim_mlp* h) {
 }
 
 // Layout of the no-allocation forward pass in the scratch array for `rows` rows (a multiple of
-// BATCH_ROUND): every piece starts on a 256-byte boundary, as hipMalloc'd buffers do, so the GEMM
-// takes the same vector paths as in psim_mlp_forward.
+// BATCH_ROUND): every piece starts on a 256-byte boundary, as hipMalloc'd buffers do, so the
+// kernels take the same vector paths as in psim_mlp_forward.
 struct NoallocLayout {
-    size_t xin, a0, act0, act1, z, y, end;
+    size_t xin, a0, act0, act1, zout, y, end;
 };
 
 NoallocLayout noalloc_layout(const psim_mlp* h, int64_t rows) {
@@ -545,8 +1091,8 @@ NoallocLayout noalloc_layout(const psim_mlp* h, int64_t rows) {
     at = up(at + (size_t)rows * wide * sizeof(uint16_t));
     l.act1 = at;
     at = up(at + (size_t)rows * wide * sizeof(uint16_t));
-    l.z = at;
-    at = up(at + (size_t)rows * wide * sizeof(float));
+    l.zout = at;
+    at = up(at + (size_t)rows * PAD_OUT * sizeof(float));
     l.y = at;
     at = up(at + (size_t)rows * sizeof(float));
     l.end = at;
@@ -582,7 +1128,7 @@ int read_loss(psim_mlp* h, int rows, float* loss) {
 extern "C" {
 
 int psim_mlp_create(psim_mlp** out, int device, int32_t n_in, int32_t hidden, int32_t layers) {
-    if (!out || n_in <= 0 || hidden <= 0 || layers <= 0 || n_in > 4096 || hidden > 8192 || layers > 16) {
+    if (!out || n_in <= 0 || hidden <= 0 || layers <= 0 || n_in > 4096 || hidden > 8192 || layers > MAX_WEIGHTS - 1) {
         return PSIM_E_INVALID;
     }
     *out = nullptr;
@@ -617,7 +1163,8 @@ int psim_mlp_create(psim_mlp** out, int device, int32_t n_in, int32_t hidden, in
     for (size_t l = 0; l + 1 < h->dims.size(); ++l) { widest = std::max(widest, (size_t)h->dims[l] * h->dims[l + 1]); }
     h->ws_elems = (size_t)MAX_SPLITS * widest;
     h->ws = mlp_alloc<float>(h->ws_elems);
-    if (!h->p || !h->g || !h->m1 || !h->m2 || !h->mask || !h->pb || !h->pbT || !h->ws) {
+    h->counters = mlp_alloc<unsigned>(NT_COUNTERS);
+    if (!h->p || !h->g || !h->m1 || !h->m2 || !h->mask || !h->pb || !h->pbT || !h->ws || !h->counters) {
         psim_mlp_destroy(h);
         return PSIM_E_DEVICE;
     }
@@ -633,7 +1180,8 @@ int psim_mlp_create(psim_mlp** out, int device, int32_t n_in, int32_t hidden, in
         || hipMemset(h->m1, 0, sizeof(float) * (size_t)off) != hipSuccess
         || hipMemset(h->m2, 0, sizeof(float) * (size_t)off) != hipSuccess
         || hipMemset(h->pb, 0, sizeof(uint16_t) * (size_t)off) != hipSuccess
-        || hipMemset(h->pbT, 0, sizeof(uint16_t) * (size_t)off) != hipSuccess) {
+        || hipMemset(h->pbT, 0, sizeof(uint16_t) * (size_t)off) != hipSuccess
+        || hipMemset(h->counters, 0, sizeof(unsigned) * NT_COUNTERS) != hipSuccess) {
         psim_mlp_destroy(h);
         return PSIM_E_DEVICE;
     }
@@ -649,8 +1197,9 @@ void psim_mlp_destroy(psim_mlp* h) {
     if (!h) { return; }
     (void)hipSetDevice(h->device);
     if (h->stream) { (void)hipStreamSynchronize(h->stream); }
+    free_train(h);
     free_batch(h);
-    void* ps[] = { h->p, h->g, h->m1, h->m2, h->mask, h->pb, h->pbT, h->ws };
+    void* ps[] = { h->p, h->g, h->m1, h->m2, h->mask, h->pb, h->pbT, h->ws, h->counters };
     for (void* q : ps) {
         if (q) { (void)hipFree(q); }
     }
@@ -673,11 +1222,9 @@ int psim_mlp_set_params(psim_mlp* h, const float* params) {
     }
     for (int n = 1; n < PAD_OUT; ++n) { p[(size_t)(h->b_off[L - 1] + n)] = 0.f; }
     MLP_CHECK(h, hipMemcpy(h->p, p.data(), sizeof(float) * p.size(), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(to_bf16_kernel, dim3(ew_blocks(h->n_params)), dim3(EW_BLOCK), 0, h->stream, h->p, h->pb,
-        h->n_params);
+    hipLaunchKernelGGL(params_bf16_kernel, dim3(ew_blocks(h->n_params)), dim3(EW_BLOCK), 0, h->stream, h->p, h->pb,
+        h->pbT, layer_table(h), h->n_params);
     MLP_CHECK(h, hipGetLastError());
-    int rc;
-    if ((rc = refresh_transposed(h)) != 0) { return rc; }
     MLP_CHECK(h, hipMemsetAsync(h->m1, 0, sizeof(float) * (size_t)h->n_params, h->stream));
     MLP_CHECK(h, hipMemsetAsync(h->m2, 0, sizeof(float) * (size_t)h->n_params, h->stream));
     MLP_CHECK(h, hipStreamSynchronize(h->stream));
@@ -699,10 +1246,9 @@ int psim_mlp_forward(psim_mlp* h, const float* x, int32_t rows, float* y) {
     MLP_CHECK(h, hipSetDevice(h->device));
     int rc;
     if ((rc = load_batch(h, x, nullptr, rows)) != 0) { return rc; }
-    if ((rc = forward(h, rows)) != 0) { return rc; }
-    const int L = (int)h->dims.size() - 1;
+    if ((rc = forward(h, rows, false)) != 0) { return rc; }
     std::vector<float> z((size_t)rows * PAD_OUT);
-    MLP_CHECK(h, hipMemcpyAsync(z.data(), h->z[L - 1], sizeof(float) * z.size(), hipMemcpyDeviceToHost, h->stream));
+    MLP_CHECK(h, hipMemcpyAsync(z.data(), h->zout, sizeof(float) * z.size(), hipMemcpyDeviceToHost, h->stream));
     MLP_CHECK(h, hipStreamSynchronize(h->stream));
     for (int r = 0; r < rows; ++r) { y[r] = z[(size_t)r * PAD_OUT]; }
     return PSIM_OK;
@@ -721,29 +1267,21 @@ int psim_mlp_forward_noalloc(psim_mlp* h, const float* x, int32_t rows, float* y
     const int64_t cap = ((int64_t)rows + BATCH_ROUND - 1) / BATCH_ROUND * BATCH_ROUND;
     const NoallocLayout lay = noalloc_layout(h, cap);
     float* xin = reinterpret_cast<float*>(h->scratch + lay.xin);
-    uint16_t* act[3] = { reinterpret_cast<uint16_t*>(h->scratch + lay.a0),
-        reinterpret_cast<uint16_t*>(h->scratch + lay.act0), reinterpret_cast<uint16_t*>(h->scratch + lay.act1) };
-    float* z = reinterpret_cast<float*>(h->scratch + lay.z);
+    uint16_t* act0 = reinterpret_cast<uint16_t*>(h->scratch + lay.act0);
+    uint16_t* act1 = reinterpret_cast<uint16_t*>(h->scratch + lay.act1);
+    float* zout = reinterpret_cast<float*>(h->scratch + lay.zout);
     float* yd = reinterpret_cast<float*>(h->scratch + lay.y);
-    // the kernels and launch shapes of forward(), with one pre-activation buffer and two
-    // activation buffers used in turn
-    MLP_CHECK(h, hipMemcpyAsync(xin, x, sizeof(float) * (size_t)rows * h->n_in, hipMemcpyHostToDevice, h->stream));
-    hipLaunchKernelGGL(to_bf16_kernel, dim3(ew_blocks((int64_t)rows * h->n_in)), dim3(EW_BLOCK), 0, h->stream, xin,
-        act[0], (int64_t)rows * h->n_in);
-    MLP_CHECK(h, hipGetLastError());
+    // the kernels of forward(), with two activation buffers used in turn; tile shapes already
+    // tuned by an earlier call of the same shape, else the untimed default (a tile never changes a
+    // result), so no events are created here
     const int L = (int)h->dims.size() - 1;
-    const uint16_t* in = act[0];
-    for (int l = 0; l < L; ++l) {
-        const int K = h->dims[l], N = h->dims[l + 1];
-        uint16_t* out = act[1 + (l % 2)];
-        int rc;
-        if ((rc = gemm<false, true>(h, in, h->pbT + h->w_off[l], z, rows, N, K)) != 0) { return rc; }
-        hipLaunchKernelGGL(bias_act_kernel, dim3(ew_blocks((int64_t)rows * N)), dim3(EW_BLOCK), 0, h->stream, z,
-            h->p + h->b_off[l], out, (int64_t)rows, N, l + 1 < L ? 1 : 0);
-        MLP_CHECK(h, hipGetLastError());
-        in = out;
-    }
-    hipLaunchKernelGGL(column0_kernel, dim3(ew_blocks(rows)), dim3(EW_BLOCK), 0, h->stream, z, yd, rows, PAD_OUT);
+    std::vector<uint16_t*> a((size_t)L);
+    a[0] = reinterpret_cast<uint16_t*>(h->scratch + lay.a0);
+    for (int l = 1; l < L; ++l) { a[(size_t)l] = (l % 2) ? act0 : act1; }
+    MLP_CHECK(h, hipMemcpyAsync(xin, x, sizeof(float) * (size_t)rows * h->n_in, hipMemcpyHostToDevice, h->stream));
+    int rc;
+    if ((rc = forward_pass(h, rows, xin, a.data(), nullptr, zout, false)) != 0) { return rc; }
+    hipLaunchKernelGGL(column0_kernel, dim3(ew_blocks(rows)), dim3(EW_BLOCK), 0, h->stream, zout, yd, rows, PAD_OUT);
     MLP_CHECK(h, hipGetLastError());
     MLP_CHECK(h, hipMemcpyAsync(y, yd, sizeof(float) * (size_t)rows, hipMemcpyDeviceToHost, h->stream));
     MLP_CHECK(h, hipStreamSynchronize(h->stream));
@@ -756,7 +1294,7 @@ int psim_mlp_gradients(psim_mlp* h, const float* x, const float* t, int32_t rows
     MLP_CHECK(h, hipSetDevice(h->device));
     int rc;
     if ((rc = load_batch(h, x, t, rows)) != 0) { return rc; }
-    if ((rc = forward(h, rows)) != 0) { return rc; }
+    if ((rc = forward(h, rows, true)) != 0) { return rc; }
     if ((rc = backward(h, rows)) != 0) { return rc; }
     if (loss && (rc = read_loss(h, rows, loss)) != 0) { return rc; }
     if (grad) {
@@ -772,17 +1310,83 @@ int psim_mlp_train_step(psim_mlp* h, const float* x, const float* t, int32_t row
     MLP_CHECK(h, hipSetDevice(h->device));
     int rc;
     if ((rc = load_batch(h, x, t, rows)) != 0) { return rc; }
-    if ((rc = forward(h, rows)) != 0) { return rc; }
+    if ((rc = forward(h, rows, true)) != 0) { return rc; }
     if ((rc = backward(h, rows)) != 0) { return rc; }
     ++h->step;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.f - std::pow(b1, (float)h->step), c2 = 1.f - std::pow(b2, (float)h->step);
     hipLaunchKernelGGL(adam_kernel, dim3(ew_blocks(h->n_params)), dim3(EW_BLOCK), 0, h->stream, h->p, h->g, h->m1, h->m2,
-        h->mask, h->pb, h->n_params, lr, b1, b2, 1e-8f, c1, c2);
+        h->mask, h->pb, h->pbT, layer_table(h), h->n_params, lr, b1, b2, 1e-8f, c1, c2);
     MLP_CHECK(h, hipGetLastError());
-    if ((rc = refresh_transposed(h)) != 0) { return rc; }
     if (loss) { return read_loss(h, rows, loss); }
     MLP_CHECK(h, hipStreamSynchronize(h->stream));
+    return PSIM_OK;
+}
+
+int psim_mlp_time_product(psim_mlp* h, int32_t kind, int32_t rows, int32_t reps, float* ms_per_launch, int32_t* tile) {
+    if (!h || !ms_per_launch || rows <= 0 || reps <= 0 || kind < 0 || kind > 3 || h->layers < 2) {
+        return PSIM_E_INVALID;
+    }
+    if (rows > MAX_ROWS) { return mlp_fail(h, PSIM_E_INVALID, "more than 4194240 rows in one call"); }
+    MLP_CHECK(h, hipSetDevice(h->device));
+    int rc;
+    if ((rc = kind == 0 ? ensure_batch(h, rows) : ensure_train(h, rows)) != 0) { return rc; }
+    // layer 1 (hidden x hidden) of the network, on its own buffers (contents as left by earlier calls)
+    const int l = 1, K = h->dims[l], N = h->dims[l + 1];
+    NtArgs a{};
+    if (kind <= 1) {
+        a.A = h->a[l];
+        a.B = h->pbT + h->w_off[l];
+        a.M = rows;
+        a.N = N;
+        a.K = K;
+        a.bias = h->p + h->b_off[l];
+        a.relu = 1;
+        a.out = h->a[l + 1];
+        a.out_t = kind == 1 ? h->aT[l + 1] : nullptr;
+    } else if (kind == 2) {
+        a.A = h->aT[l];
+        a.B = h->dzT;
+        a.M = K;
+        a.N = N;
+        a.K = rows;
+        a.c = h->g + h->w_off[l];
+        a.ws = h->ws;
+        a.counters = h->counters;
+    } else {
+        a.A = h->dz;
+        a.B = h->pb + h->w_off[l];
+        a.M = rows;
+        a.N = K;
+        a.K = N;
+        a.mask = h->a[l];
+        a.out = h->dz2;
+        a.out_t = h->dz2T;
+    }
+    const auto launch = [&]() {
+        if (kind <= 1) { return launch_nt<EPI_FWD>(h->device, a, h->stream, true, 0); }
+        if (kind == 2) { return launch_nt<EPI_F32>(h->device, a, h->stream, true, h->ws_elems); }
+        return launch_nt<EPI_BACK>(h->device, a, h->stream, true, 0);
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MLP_CHECK(h, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return mlp_fail(h, PSIM_E_DEVICE, "hipEventCreate failed");
+    }
+    int used = launch();
+    used = launch();
+    (void)hipEventRecord(e0, h->stream);
+    for (int r = 0; r < reps; ++r) { launch(); }
+    (void)hipEventRecord(e1, h->stream);
+    float ms = 0.f;
+    const bool ok = hipGetLastError() == hipSuccess && hipEventSynchronize(e1) == hipSuccess
+                    && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!ok) { return mlp_fail(h, PSIM_E_DEVICE, "timing the product failed"); }
+    *ms_per_launch = ms / (float)reps;
+    if (tile) { *tile = NT_TILES[used].bm * 1000 + NT_TILES[used].bn; }
     return PSIM_OK;
 }
 
@@ -798,31 +1402,34 @@ int psim_mlp_gemm_check(int device, int32_t ta, int32_t tb, int32_t M, int32_t N
     for (size_t i = 0; i < nb; ++i) { hb[i] = f32_to_bf16(B[i]); }
     uint16_t *da = nullptr, *db = nullptr;
     float *dc = nullptr, *ws = nullptr;
+    unsigned* counters = nullptr;
     const size_t ws_elems = (size_t)MAX_SPLITS * nc;  // products split over K are checked too
     int rc = PSIM_OK;
     if (hipMalloc(&da, na * 2) != hipSuccess || hipMalloc(&db, nb * 2) != hipSuccess
-        || hipMalloc(&dc, nc * 4) != hipSuccess || hipMalloc(&ws, ws_elems * 4) != hipSuccess) {
+        || hipMalloc(&dc, nc * 4) != hipSuccess || hipMalloc(&ws, ws_elems * 4) != hipSuccess
+        || hipMalloc(&counters, NT_COUNTERS * sizeof(unsigned)) != hipSuccess
+        || hipMemset(counters, 0, NT_COUNTERS * sizeof(unsigned)) != hipSuccess) {
         rc = PSIM_E_DEVICE;
     } else if (hipMemcpy(da, ha.data(), na * 2, hipMemcpyHostToDevice) != hipSuccess
                || hipMemcpy(db, hb.data(), nb * 2, hipMemcpyHostToDevice) != hipSuccess) {
         rc = PSIM_E_DEVICE;
     } else {
-        launch_gemm(ta != 0, tb != 0, da, db, dc, M, N, K, 0, ws, ws_elems);
+        launch_gemm(device, ta != 0, tb != 0, da, db, dc, M, N, K, 0, ws, ws_elems, counters, true);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess
             || hipMemcpy(C, dc, nc * 4, hipMemcpyDeviceToHost) != hipSuccess) {
             rc = PSIM_E_DEVICE;
         }
     }
-    if (da) { (void)hipFree(da); }
-    if (db) { (void)hipFree(db); }
-    if (dc) { (void)hipFree(dc); }
-    if (ws) { (void)hipFree(ws); }
+    void* bufs[] = { da, db, dc, ws, counters };
+    for (void* q : bufs) {
+        if (q) { (void)hipFree(q); }
+    }
     return rc;
 }
 
-// Average duration of one GEMM of the network for op(A)[M][K] x op(B)[K][N] (a product split over
-// K includes its split reduce), from HIP events around `reps` products on a stream of its own
-// (operands filled on the device).
+// Average duration of one product op(A)[M][K] x op(B)[K][N] with an FP32 result (a product split
+// over K includes its ordered sum of the partials), from HIP events around `reps` products on a
+// stream of its own (operands filled on the device).
 int psim_mlp_time_gemm(int device, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, int32_t reps,
     float* ms_per_launch) {
     if (!ms_per_launch || M <= 0 || N <= 0 || K <= 0 || reps <= 0 || M > MAX_ROWS) { return PSIM_E_INVALID; }
@@ -830,6 +1437,7 @@ int psim_mlp_time_gemm(int device, int32_t ta, int32_t tb, int32_t M, int32_t N,
     const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
     uint16_t *da = nullptr, *db = nullptr;
     float *dc = nullptr, *src = nullptr, *ws = nullptr;
+    unsigned* counters = nullptr;
     const size_t ws_elems = (size_t)MAX_SPLITS * nc;
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -837,13 +1445,14 @@ int psim_mlp_time_gemm(int device, int32_t ta, int32_t tb, int32_t M, int32_t N,
     const size_t nsrc = std::max(na, nb);
     if (hipMalloc(&da, na * 2) != hipSuccess || hipMalloc(&db, nb * 2) != hipSuccess
         || hipMalloc(&dc, nc * 4) != hipSuccess || hipMalloc(&src, nsrc * 4) != hipSuccess
-        || hipMalloc(&ws, ws_elems * 4) != hipSuccess || hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess
-        || hipEventCreate(&e1) != hipSuccess) {
+        || hipMalloc(&ws, ws_elems * 4) != hipSuccess || hipMalloc(&counters, NT_COUNTERS * sizeof(unsigned)) != hipSuccess
+        || hipMemset(counters, 0, NT_COUNTERS * sizeof(unsigned)) != hipSuccess || hipStreamCreate(&st) != hipSuccess
+        || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
         rc = PSIM_E_DEVICE;
     } else {
         std::vector<float> h(nsrc);
         for (size_t i = 0; i < nsrc; ++i) { h[i] = (float)((i * 2654435761u) % 1000) * 1e-3f - 0.5f; }
-        auto launch = [&] { launch_gemm(ta != 0, tb != 0, da, db, dc, M, N, K, st, ws, ws_elems); };
+        auto launch = [&] { launch_gemm(device, ta != 0, tb != 0, da, db, dc, M, N, K, st, ws, ws_elems, counters, true); };
         float ms = 0.f;
         if (hipMemcpy(src, h.data(), nsrc * 4, hipMemcpyHostToDevice) != hipSuccess) {
             rc = PSIM_E_DEVICE;
@@ -865,7 +1474,7 @@ int psim_mlp_time_gemm(int device, int32_t ta, int32_t tb, int32_t M, int32_t N,
     if (e0) { (void)hipEventDestroy(e0); }
     if (e1) { (void)hipEventDestroy(e1); }
     if (st) { (void)hipStreamDestroy(st); }
-    void* bufs[] = { da, db, dc, src, ws };
+    void* bufs[] = { da, db, dc, src, ws, counters };
     for (void* q : bufs) {
         if (q) { (void)hipFree(q); }
     }

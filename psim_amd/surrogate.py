@@ -51,6 +51,8 @@ def _bind():
                                                ctypes.c_int32, ctypes.c_int32, _F, _F, _F]),
         "psim_mlp_time_gemm": (ctypes.c_int, [ctypes.c_int, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                               ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _F]),
+        "psim_mlp_time_product": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _F,
+                                                 ctypes.POINTER(ctypes.c_int32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -163,6 +165,17 @@ class Surrogate:
         loss = ctypes.c_float()
         self._check(self._L.psim_mlp_gradients(self._h, _fp(x), _fp(t), x.shape[0], _fp(g), ctypes.byref(loss)))
         return loss.value, g
+
+    PRODUCTS = {"forward": 0, "forward_train": 1, "weight_grad": 2, "input_grad": 3}
+
+    def time_product(self, kind: str, rows: int, reps: int = 50) -> tuple:
+        """(ms per launch, tile) of one of the network's fused hidden x hidden products at `rows` rows
+        (psim_mlp_time_product): forward, forward_train, weight_grad or input_grad."""
+        ms = ctypes.c_float()
+        tile = ctypes.c_int32()
+        self._check(self._L.psim_mlp_time_product(self._h, self.PRODUCTS[kind], int(rows), int(reps), ctypes.byref(ms),
+                                                  ctypes.byref(tile)))
+        return float(ms.value), f"{tile.value // 1000}x{tile.value % 1000}"
 
     def train_step(self, x: np.ndarray, t: np.ndarray, lr: float) -> float:
         x = np.ascontiguousarray(x, np.float32)

@@ -162,3 +162,33 @@ def test_too_many_rows_are_rejected(surrogate):
     assert rc != 0
     assert "rows" in L.psim_mlp_last_error(net._h).decode()
     np.testing.assert_array_equal(net.predict(x), net.predict(x))  # the handle still works
+
+
+def test_every_tile_and_split_sum_gives_the_same_bits(surrogate, monkeypatch):
+    """The autotuner may pick any tile shape of the K-major kernel, and for a product split over K
+    either the in-kernel or the separate ordered sum of the partials (PSIM_GEMM_CHOICE forces each
+    in turn): a forward product, a split weight-gradient-shaped product and the network's forward
+    and gradients must come out bit-identical under every choice."""
+    rng = np.random.default_rng(21)
+    a = rng.normal(size=(1024, 512)).astype(np.float32)
+    b = rng.normal(size=(512, 512)).astype(np.float32)
+    aw = rng.normal(size=(512, 4096)).astype(np.float32)
+    bw = rng.normal(size=(512, 4096)).astype(np.float32)
+    # 2048 rows: the hidden layers' weight gradients (K = rows) are split over K
+    x = rng.uniform(-1, 1, size=(2048, surrogate.N_FEATURES)).astype(np.float32)
+    t = rng.uniform(-1, 1, size=2048).astype(np.float32)
+    net = surrogate.Surrogate(seed=4)
+    results = []
+    for choice in range(10):
+        monkeypatch.setenv("PSIM_GEMM_CHOICE", str(choice))
+        fwd = surrogate.gemm_check(a, b, False, True)
+        split = surrogate.gemm_check(aw, bw, False, True)  # 512 x 512 x 4096: split over K
+        y = net.predict(x)
+        loss, grad = net.gradients(x, t)
+        results.append((fwd, split, y, grad))
+    net.close()
+    for fwd, split, y, grad in results[1:]:
+        np.testing.assert_array_equal(fwd, results[0][0])
+        np.testing.assert_array_equal(split, results[0][1])
+        np.testing.assert_array_equal(y, results[0][2])
+        np.testing.assert_array_equal(grad, results[0][3])

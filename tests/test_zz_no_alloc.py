@@ -42,12 +42,12 @@ def test_noalloc_forward_equals_forward():
 
 
 def test_noalloc_forward_takes_its_stated_maximum_and_refuses_more():
-    """15616 rows fit the 64 MiB array for 32 -> 512 x 4 (include/psim_surrogate.h); 15617 do not."""
+    """29056 rows fit the 64 MiB array for 32 -> 512 x 4 (include/psim_surrogate.h); 29057 do not."""
     net = surrogate.Surrogate(seed=3)
-    x = np.random.default_rng(5).uniform(-1.0, 1.0, size=(15617, surrogate.N_FEATURES)).astype(np.float32)
-    rc, y = _noalloc(net, x[:15616])
+    x = np.random.default_rng(5).uniform(-1.0, 1.0, size=(29057, surrogate.N_FEATURES)).astype(np.float32)
+    rc, y = _noalloc(net, x[:29056])
     assert rc == 0, net._L.psim_mlp_last_error(net._h).decode()
-    np.testing.assert_array_equal(y, net.predict(x[:15616]))
+    np.testing.assert_array_equal(y, net.predict(x[:29056]))
     rc, _ = _noalloc(net, x)
     assert rc != 0
     assert "scratch" in net._L.psim_mlp_last_error(net._h).decode()
