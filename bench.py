@@ -277,13 +277,15 @@ def batch_bench(args) -> dict:
     # launch shape for this batch (the batch is now uploaded); no choice changes any result
     tune = None if args.no_autotune else engine.autotune(seed=20_000)
     gpu_s = wall = 0.0
-    steps_total = 0
+    steps_total = events_total = 0
     for k in range(args.steps):
         t0 = time.perf_counter()
         counters = psim_amd.run_batch(engine, models, seed=k)
         wall += time.perf_counter() - t0
         gpu_s += engine.last_timing()["run_ms"] / 1e3
         steps_total += counters["phonon_steps"]
+        # events of the event loop: measurement boundaries passed, scatters and wall impacts
+        events_total += counters["phonon_steps"] + counters["scatters"] + counters["impacts"]
     value = steps_total / gpu_s
     # the uploaded batch replayed from a captured graph (capture on the first call)
     n_sensors = sum(len(c["sensors"]) for c in cfgs)
@@ -297,6 +299,7 @@ def batch_bench(args) -> dict:
     # the same models one at a time (each its own upload and run), with the default launch shape
     engine.set_window(0)
     engine.set_grid(8)
+    engine.set_block(256)
     seq_steps, seq_gpu = 0, 0.0
     for m in ([] if args.no_sequential else models):
         m.run(engine, 0)
@@ -313,6 +316,8 @@ def batch_bench(args) -> dict:
                    "tallies": "fp64", "parallelism": "single GPU"},
         "batch": {"gpu_run_s": gpu_s / args.steps, "end_to_end_s": wall / args.steps,
                   "host_model_build_s": t_build, "phonon_steps": steps_total // args.steps,
+                  "events_per_s": events_total / gpu_s,
+                  "events_per_phonon_step": events_total / steps_total,
                   "autotune": tune, "graph_replay": {"value": g_steps / g_s, "gpu_run_s": g_s / args.steps}},
         "sequential_gpu": None if args.no_sequential else {
             "value": seq_steps / seq_gpu, "gpu_run_s": seq_gpu, "phonon_steps": seq_steps,

@@ -232,6 +232,14 @@ int psim_gpu_set_window(psim_gpu_ctx* ctx, int32_t steps_per_launch);
 /* Engine tuning: transport blocks per compute unit (default 8). */
 int psim_gpu_set_grid(psim_gpu_ctx* ctx, int32_t blocks_per_cu);
 
+/* Engine tuning: threads per transport block, 128, 256 (default) or 512.  No size changes a
+ * result.  No counterpart in the reference. */
+int psim_gpu_set_block(psim_gpu_ctx* ctx, int32_t threads);
+
+/* The launch shape in use: steps per untallied launch (0 = automatic), transport blocks per CU
+ * and threads per transport block (any pointer may be NULL). */
+int psim_gpu_get_launch_shape(const psim_gpu_ctx* ctx, int32_t* window, int32_t* blocks_per_cu, int32_t* block);
+
 /* Engine tuning: where recorded steps are tallied.  AUTO chooses by model size; LDS keeps
  * per-block window partials in LDS (used whenever one step of tallies fits); HBM adds into the
  * tallies with FP64 atomics in one launch.  Integer counts are the same either way; flux sums
@@ -257,8 +265,10 @@ int psim_gpu_run_graph(psim_gpu_ctx* ctx, uint64_t seed, int64_t* counts, double
 int psim_gpu_run_async(psim_gpu_ctx* ctx, uint64_t seed, void* stream);
 
 /* Autotuner: runs the uploaded model once for each candidate launch shape (untallied window
- * length 0 / 1024 / 256, 8 or 3 transport blocks per CU; none changes any result), keeps the
- * fastest in the context and reports it.  No counterpart in the reference. */
+ * length 0 / 1024 / 256 and 8 or 3 transport blocks per CU at the current block size, then 128,
+ * 256 and 512 threads per block at the best of those; none changes any result), keeps the fastest
+ * in the context and reports it (the block size through psim_gpu_get_launch_shape).  No
+ * counterpart in the reference. */
 int psim_gpu_autotune(psim_gpu_ctx* ctx, uint64_t seed, int32_t* window, int32_t* blocks_per_cu, double* best_ms);
 
 /* ---- host model (C++17 behind the C ABI) ------------------------------------ */

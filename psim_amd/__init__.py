@@ -68,6 +68,9 @@ def lib():
         "psim_gpu_last_timing": (ctypes.c_int, [_VP, _D, _I64, _D]),
         "psim_gpu_set_window": (ctypes.c_int, [_VP, ctypes.c_int32]),
         "psim_gpu_set_grid": (ctypes.c_int, [_VP, ctypes.c_int32]),
+        "psim_gpu_set_block": (ctypes.c_int, [_VP, ctypes.c_int32]),
+        "psim_gpu_get_launch_shape": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                                     ctypes.POINTER(ctypes.c_int32)]),
         "psim_gpu_run_graph": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int64),
                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
         "psim_gpu_run_async": (ctypes.c_int, [_VP, ctypes.c_uint64, _VP]),
@@ -144,6 +147,15 @@ class Engine:
     def set_grid(self, blocks_per_cu: int) -> None:
         self._check(self._L.psim_gpu_set_grid(self._h, int(blocks_per_cu)))
 
+    def set_block(self, threads: int) -> None:
+        """Threads per transport block: 128, 256 (default) or 512."""
+        self._check(self._L.psim_gpu_set_block(self._h, int(threads)))
+
+    def launch_shape(self) -> dict:
+        w, g, b = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self._check(self._L.psim_gpu_get_launch_shape(self._h, ctypes.byref(w), ctypes.byref(g), ctypes.byref(b)))
+        return {"window": w.value, "blocks_per_cu": g.value, "block": b.value}
+
     def set_tally_mode(self, mode: str) -> None:
         """'auto', 'lds' or 'hbm': where recorded steps are tallied (psim_gpu_set_tally_mode)."""
         codes = {"auto": 0, "lds": 1, "hbm": 2}
@@ -173,7 +185,7 @@ class Engine:
         g = ctypes.c_int32()
         ms = ctypes.c_double()
         self._check(self._L.psim_gpu_autotune(self._h, int(seed), ctypes.byref(w), ctypes.byref(g), ctypes.byref(ms)))
-        return {"window": w.value, "blocks_per_cu": g.value, "run_ms": ms.value}
+        return {"window": w.value, "blocks_per_cu": g.value, "block": self.launch_shape()["block"], "run_ms": ms.value}
 
     def fetch(self, n_sensors: int, num_record: int):
         counts = np.zeros((n_sensors, num_record), np.int64)

@@ -305,8 +305,9 @@ enum : int { S_FETCH = 0, S_TTS = 1, S_NI = 2, S_COLL = 3, S_SCAT = 4, S_END = 5
 // id within its stream is its id and the stream word is the bare tag, so the single-model
 // variant neither loads nor stores the per-phonon stream id and keeps that word a constant;
 // both variants draw the same numbers for such a model.
-template <int TALLY, bool MULTI>
-__global__ void __launch_bounds__(BLOCK)
+// NT: threads per block (128, 256 or 512), a launch shape the autotuner chooses.
+template <int TALLY, bool MULTI, int NT>
+__global__ void __launch_bounds__(NT)
 __attribute__((amdgpu_waves_per_eu(TALLY == TALLY_NONE ? PSIM_UNTALLIED_WAVES_PER_EU : PSIM_TRANSPORT_WAVES_PER_EU)))
 transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const uint64_t* __restrict__ seed_p,
     Tally tally, unsigned long long* __restrict__ counters, Fate fate) {
@@ -321,7 +322,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     double* lfx = reinterpret_cast<double*>(lds_raw);
     double* lfy = lfx + WS1;
     int32_t* lcnt = reinterpret_cast<int32_t*>(lfy + WS1);
-    for (int i = threadIdx.x; i < WS1; i += BLOCK) {
+    for (int i = threadIdx.x; i < WS1; i += NT) {
         lfx[i] = 0.;
         lfy[i] = 0.;
         lcnt[i] = 0;
@@ -745,7 +746,7 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     if (WS) {  // the difference arrays summed over the window's steps, per sensor, into the partials
         __syncthreads();
         const int64_t base = (int64_t)blockIdx.x * WS;
-        for (int sn = threadIdx.x; sn < S; sn += BLOCK) {
+        for (int sn = threadIdx.x; sn < S; sn += NT) {
             int32_t cnt = 0;
             double fx = 0., fy = 0.;
             for (int e = sn; e < WS; e += S) {
@@ -973,6 +974,7 @@ struct psim_gpu_ctx {
     int64_t steps_cap = 0;
     int32_t window = 0;
     int32_t blocks_per_cu = 8;  // transport grid: n_CU x blocks_per_cu blocks
+    int32_t block = 256;  // threads per transport block
     int32_t tally_mode = PSIM_TALLY_AUTO;  // where recorded steps are tallied
     // seed of the current run, read by the kernels from device memory (pinned host staging copy)
     uint64_t* d_seed = nullptr;
@@ -1176,18 +1178,19 @@ int prepare_run(psim_gpu_ctx* c, RunPlan& plan) {
         c->fate_cap = (int64_t)cap;
         c->gen++;
     }
-    const int64_t blocks_needed = (n + BLOCK - 1) / BLOCK;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(blocks_needed, (int64_t)c->n_cu * c->blocks_per_cu));
+    const int64_t blocks_needed = (n + BLOCK - 1) / BLOCK;  // emission and birth sort: one thread per phonon
+    const int tb = c->block;  // transport: c->block threads per block, n_CU x blocks_per_cu blocks at most
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + tb - 1) / tb, (int64_t)c->n_cu * c->blocks_per_cu));
     // Each lane counts its phonon-steps, tallies and events in 32 bits until the launch ends (the
     // wave sums are 64-bit).  The work queue spreads the phonons evenly over the lanes, so a lane
     // carries about n * N / lanes phonon-steps at most; runs where that average comes within a
     // factor of 2 of 2^32 are refused rather than risk a wrapped counter.
-    const double lane_share = (double)n * (double)P.num_measurements / ((double)grid * BLOCK);
+    const double lane_share = (double)n * (double)P.num_measurements / ((double)grid * tb);
     if (lane_share >= 2147483648.0) {
         return fail(c, PSIM_E_INVALID, "run too large: phonons x measurement steps per lane exceeds 2^31");
     }
     // A live buffer holds every phonon plus the survivor slots the waves may reserve but not fill.
-    const int64_t live_need = n + (int64_t)OUT_CLAIM * grid * (BLOCK / WAVE);
+    const int64_t live_need = n + (int64_t)OUT_CLAIM * grid * (tb / WAVE);
     if (c->list_cap < live_need) {
         free_lists(c);
         const size_t cap = (size_t)std::max<int64_t>(live_need, 1);
@@ -1268,12 +1271,13 @@ int zero_fill(psim_gpu_ctx* c, void* p, size_t bytes) {
     return PSIM_OK;
 }
 
-// One transport launch of the variant for this tally path and stream layout.
-void launch_transport(psim_gpu_ctx* c, int tally, int grid, size_t lds, const Lists& lists, int32_t k0, int32_t k1,
+// One transport launch of the variant for this tally path, stream layout and block size.
+template <int NT>
+void launch_transport_nt(psim_gpu_ctx* c, int tally, int grid, size_t lds, const Lists& lists, int32_t k0, int32_t k1,
     const Tally& t, const Fate& fate) {
-    const dim3 g((unsigned)grid), b(BLOCK);
+    const dim3 g((unsigned)grid), b(NT);
 #define PSIM_TRANSPORT(T, M)                                                                                 \
-    hipLaunchKernelGGL((transport_kernel<T, M>), g, b, lds, c->stream, c->m, c->pool, lists, k0, k1, c->d_seed, t, \
+    hipLaunchKernelGGL((transport_kernel<T, M, NT>), g, b, lds, c->stream, c->m, c->pool, lists, k0, k1, c->d_seed, t, \
         c->d_counters, fate)
     if (c->multi_stream) {
         if (tally == TALLY_NONE) {
@@ -1291,6 +1295,17 @@ void launch_transport(psim_gpu_ctx* c, int tally, int grid, size_t lds, const Li
         PSIM_TRANSPORT(TALLY_LDS, false);
     }
 #undef PSIM_TRANSPORT
+}
+
+void launch_transport(psim_gpu_ctx* c, int tally, int grid, size_t lds, const Lists& lists, int32_t k0, int32_t k1,
+    const Tally& t, const Fate& fate) {
+    if (c->block == 128) {
+        launch_transport_nt<128>(c, tally, grid, lds, lists, k0, k1, t, fate);
+    } else if (c->block == 512) {
+        launch_transport_nt<512>(c, tally, grid, lds, lists, k0, k1, t, fate);
+    } else {
+        launch_transport_nt<256>(c, tally, grid, lds, lists, k0, k1, t, fate);
+    }
 }
 
 // Every launch of one run, on c->stream.  `eager`: also times each transport window with events
@@ -1746,6 +1761,21 @@ int psim_gpu_set_grid(psim_gpu_ctx* c, int32_t blocks_per_cu) {
     return PSIM_OK;
 }
 
+int psim_gpu_set_block(psim_gpu_ctx* c, int32_t threads) {
+    if (!c || (threads != 128 && threads != 256 && threads != 512)) { return PSIM_E_INVALID; }
+    if (c->block != threads) { c->gen++; }
+    c->block = threads;
+    return PSIM_OK;
+}
+
+int psim_gpu_get_launch_shape(const psim_gpu_ctx* c, int32_t* window, int32_t* blocks_per_cu, int32_t* block) {
+    if (!c) { return PSIM_E_INVALID; }
+    if (window) { *window = c->window; }
+    if (blocks_per_cu) { *blocks_per_cu = c->blocks_per_cu; }
+    if (block) { *block = c->block; }
+    return PSIM_OK;
+}
+
 int psim_gpu_set_tally_mode(psim_gpu_ctx* c, int32_t mode) {
     if (!c || mode < PSIM_TALLY_AUTO || mode > PSIM_TALLY_HBM) { return PSIM_E_INVALID; }
     if (c->tally_mode != mode) { c->gen++; }
@@ -1780,33 +1810,50 @@ int psim_gpu_run_graph(psim_gpu_ctx* c, uint64_t seed, int64_t* counts, double* 
 int psim_gpu_autotune(psim_gpu_ctx* c, uint64_t seed, int32_t* window, int32_t* blocks_per_cu, double* best_ms) {
     if (!c) { return PSIM_E_INVALID; }
     // Launch shapes that leave every result unchanged: the length of the untallied windows
-    // (0 = one launch; shorter ones re-sort the survivors by cell more often) and the number of
-    // transport blocks per CU.  Each candidate runs the uploaded model once; the fastest is kept.
+    // (0 = one launch; shorter ones re-sort the survivors by cell more often), the number of
+    // transport blocks per CU and the threads per transport block.  Each candidate runs the
+    // uploaded model once; the fastest is kept.  Window and grid are searched at the current block
+    // size, then the other block sizes at the best window and grid.
     const int32_t windows[] = { 0, 1024, 256 };
     const int32_t grids[] = { 8, 3 };
-    const int32_t w0 = c->window, g0 = c->blocks_per_cu;
+    const int32_t blocks[] = { 128, 256, 512 };
+    const int32_t w0 = c->window, g0 = c->blocks_per_cu, b0 = c->block;
     double best = -1.;
-    int32_t bw = w0, bg = g0;
+    int32_t bw = w0, bg = g0, bb = b0;
+    auto trial = [&](int32_t w, int32_t g, int32_t b) -> int {
+        c->window = w;
+        c->blocks_per_cu = g;
+        c->block = b;
+        c->gen++;
+        const int rc = run_impl(c, seed);
+        if (rc != PSIM_OK) {
+            c->window = w0;
+            c->blocks_per_cu = g0;
+            c->block = b0;
+            c->gen++;
+            return rc;
+        }
+        if (best < 0. || c->run_ms < best) {
+            best = c->run_ms;
+            bw = w;
+            bg = g;
+            bb = b;
+        }
+        return PSIM_OK;
+    };
+    int rc;
     for (int32_t g : grids) {
         for (int32_t w : windows) {
-            c->window = w;
-            c->blocks_per_cu = g;
-            c->gen++;
-            const int rc = run_impl(c, seed);
-            if (rc != PSIM_OK) {
-                c->window = w0;
-                c->blocks_per_cu = g0;
-                return rc;
-            }
-            if (best < 0. || c->run_ms < best) {
-                best = c->run_ms;
-                bw = w;
-                bg = g;
-            }
+            if ((rc = trial(w, g, b0)) != PSIM_OK) { return rc; }
         }
+    }
+    const int32_t wb = bw, gb = bg;
+    for (int32_t b : blocks) {
+        if (b != b0 && (rc = trial(wb, gb, b)) != PSIM_OK) { return rc; }
     }
     c->window = bw;
     c->blocks_per_cu = bg;
+    c->block = bb;
     c->gen++;
     if (window) { *window = bw; }
     if (blocks_per_cu) { *blocks_per_cu = bg; }

@@ -77,10 +77,13 @@ def test_autotune_keeps_results(engine):
     choice = engine.autotune(seed=1)
     print("autotune choice", choice, flush=True)
     assert choice["window"] in (0, 1024, 256) and choice["blocks_per_cu"] in (8, 3) and choice["run_ms"] > 0
+    assert choice["block"] in (128, 256, 512)
+    assert engine.launch_shape() == {k: choice[k] for k in ("window", "blocks_per_cu", "block")}
     _same(_eager(engine, model, 3), reference)
     _same(engine.run_graph(model.num_sensors, model.num_record, 3), reference)
     engine.set_window(0)
     engine.set_grid(8)
+    engine.set_block(256)
     engine.set_tally_mode("auto")
 
 

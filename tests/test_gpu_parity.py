@@ -140,6 +140,37 @@ def test_launch_window_does_not_change_results(engine, name, window):
     assert _flux_close(res.raw_flux, base.raw_flux)
 
 
+@pytest.mark.parametrize("block", [128, 512])
+@pytest.mark.parametrize("tally", ["lds", "hbm"])
+def test_block_size_does_not_change_results(engine, block, tally):
+    """Threads per transport block (a launch shape the autotuner chooses) leave every trajectory
+    unchanged: fates, counts and counters equal the 256-thread run's, on both tally paths."""
+    cfg = CASES["bar_periodic_mixed"]
+    print(f"case bar_periodic_mixed block {block} tally {tally}", flush=True)
+    model = psim_amd.Model(cfg)
+    engine.set_fate(True)
+    engine.set_window(0)
+    engine.set_block(256)
+    engine.set_tally_mode(tally)
+    model.run(engine, 5)
+    base = model.results(0)
+    _, total = model.budget()
+    base_fate = engine.fetch_fate(total)
+    engine.set_block(block)
+    try:
+        model.run(engine, 5)
+        res = model.results(0)
+        fate = engine.fetch_fate(total)
+    finally:
+        engine.set_block(256)
+        engine.set_tally_mode("auto")
+    np.testing.assert_array_equal(res.counts, base.counts)
+    assert res.counters == base.counters
+    for a, b in zip(fate, base_fate):
+        np.testing.assert_array_equal(a, b)
+    assert _flux_close(res.raw_flux, base.raw_flux)
+
+
 def test_each_run_of_a_model_draws_from_its_own_seed(engine):
     """A model with num_runs = 2 (Model::runSimulation's loop, model.cpp:141-182): every run
     starts from reset controllers, and run r draws from seed + r * 0x9E3779B97F4A7C15, so it
