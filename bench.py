@@ -541,9 +541,12 @@ def main() -> int:
     kernel_ms = 0.0
     launches = 0
     t0 = time.perf_counter()
+    flux64 = None
     for k in range(args.steps):
         engine.run_resident(k)
-        _, _, counters = engine.fetch(model.num_sensors, model.num_record)
+        _, flux, counters = engine.fetch(model.num_sensors, model.num_record)
+        if k == 0:
+            flux64 = flux
         tm = engine.last_timing()
         steps_total += counters["phonon_steps"]
         kernel_ms += tm["step_kernel_ms"]
@@ -605,6 +608,21 @@ def main() -> int:
                 "file": "profiles/transport_kernel_lshape_1e7_counters.json",
                 "engine_commit": copied["engine_commit"], "engine_is_current": copied["current"],
                 "traffic": "bytes per phonon-step from the profile x this run's phonon-steps per launch"}
+    # The same run (seed 0) with FP32 flux tallies summed with Kahan compensation, against FP64:
+    # the same trajectories, so the difference is the FP32 tallies' error alone.
+    if flux64 is not None:
+        engine.set_tally_precision("fp32_kahan")
+        engine.run_resident(0)
+        _, flux32, c32 = engine.fetch(model.num_sensors, model.num_record)
+        ms32 = engine.last_timing()["run_ms"]
+        engine.set_tally_precision("fp64")
+        m32, m64 = flux32.mean(axis=1), flux64.mean(axis=1)  # [sensor][x, y] mean over the recorded steps
+        line["fp32_kahan_tallies"] = {
+            "value": c32["phonon_steps"] / (ms32 / 1e3), "unit": "phonon-steps/s", "run_ms": ms32,
+            "max_abs_diff_over_max_flux": float(abs(flux32 - flux64).max() / max(abs(flux64).max(), 1e-300)),
+            "max_diff_sensor_mean_flux_over_largest": float(abs(m32 - m64).max() / max(abs(m64).max(), 1e-300)),
+            "method": "FP32 atomics into the LDS/HBM difference arrays, Kahan-compensated FP32 sums along the steps "
+                      "and across blocks, returned as sum - compensation; compared with the FP64 run of the same seed"}
     # the boundary's own call end to end: host tables and budget, upload, run, tally copy-back and
     # interpretation (psim_model_run); reported beside `value`, which has the model resident in HBM
     if not film:

@@ -249,6 +249,17 @@ int psim_gpu_get_launch_shape(const psim_gpu_ctx* ctx, int32_t* window, int32_t*
 #define PSIM_TALLY_HBM 2
 int psim_gpu_set_tally_mode(psim_gpu_ctx* ctx, int32_t mode);
 
+/* Engine tuning: precision of the flux tallies.  FP64 (default) adds doubles, as the reference's
+ * Sensor::updateHeatParams does (psim/src/sensor.cpp:43-52).  FP32_KAHAN keeps the LDS window
+ * difference arrays and the HBM ones in FP32 (FP32 atomics), sums the window partials along the
+ * steps and across blocks with Kahan compensation in FP32, and returns each tally as
+ * sum - compensation in the same double arrays; integer counts are unaffected.  The launch plan
+ * and trajectories are those of FP64 (its transport blocks are 256 threads), so a run of each on
+ * one seed shows the error of the FP32 tallies against FP64.  No counterpart in the reference. */
+#define PSIM_TALLY_FP64 0
+#define PSIM_TALLY_FP32_KAHAN 1
+int psim_gpu_set_tally_precision(psim_gpu_ctx* ctx, int32_t precision);
+
 /* psim_gpu_run replayed from a HIP graph of the whole run (emission, sorts, every transport
  * window and reduction), captured on the first call and again after anything that changes the
  * launch sequence (upload, window, grid, fate recording).  The seed is data, so one graph serves

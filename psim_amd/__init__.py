@@ -75,6 +75,7 @@ def lib():
                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
         "psim_gpu_run_async": (ctypes.c_int, [_VP, ctypes.c_uint64, _VP]),
         "psim_gpu_set_tally_mode": (ctypes.c_int, [_VP, ctypes.c_int32]),
+        "psim_gpu_set_tally_precision": (ctypes.c_int, [_VP, ctypes.c_int32]),
         "psim_gpu_autotune": (ctypes.c_int, [_VP, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32),
                                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
         "psim_model_from_json": (ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
@@ -160,6 +161,11 @@ class Engine:
         """'auto', 'lds' or 'hbm': where recorded steps are tallied (psim_gpu_set_tally_mode)."""
         codes = {"auto": 0, "lds": 1, "hbm": 2}
         self._check(self._L.psim_gpu_set_tally_mode(self._h, codes[mode]))
+
+    def set_tally_precision(self, precision: str) -> None:
+        """'fp64' (default) or 'fp32_kahan': the flux tallies' arithmetic (psim_gpu_set_tally_precision)."""
+        codes = {"fp64": 0, "fp32_kahan": 1}
+        self._check(self._L.psim_gpu_set_tally_precision(self._h, codes[precision]))
 
     def run_resident(self, seed: int) -> None:
         self._check(self._L.psim_gpu_run_resident(self._h, int(seed)))

@@ -56,13 +56,16 @@ constexpr uint32_t OUT_CLAIM = 64;  // survivor slots a wave reserves per atomic
 // or every segment when even GLOBAL_TALLY_STEPS steps of S sensors do not fit the LDS budget
 // (thousands of sensors, e.g. a batch of geometries), goes to [S][R + 1] difference arrays in HBM
 // with atomics, summed into the tallies when the run ends (diff_prefix_kernel).
+// Flux arrays hold FT elements (double, or float in the FP32 + Kahan tally precision).
 struct Tally {
     int32_t* __restrict__ pcnt;  // [grid][W*S]
-    double* __restrict__ pfx;
-    double* __restrict__ pfy;
+    void* __restrict__ pfx;  // FT
+    void* __restrict__ pfy;
+    float* __restrict__ pcx;  // FP32: Kahan compensation of pfx / pfy (the negative of the lost low part)
+    float* __restrict__ pcy;
     int64_t* __restrict__ dcnt;  // [S][R + 1] (HBM difference arrays)
-    double* __restrict__ dfx;
-    double* __restrict__ dfy;
+    void* __restrict__ dfx;  // FT
+    void* __restrict__ dfy;
     int32_t rec;  // 1 if this window records
     int32_t global;  // 1: every segment into the HBM difference arrays
     int32_t W;
@@ -73,6 +76,16 @@ struct Fate {
     int32_t* __restrict__ cause;
     double* __restrict__ xy;
 };
+
+// Kahan-compensated FP32 addition: s is the running sum and c its compensation, the negative of
+// the low part s has lost, so s - c carries the sum to about twice FP32's precision (the library
+// is compiled with -ffp-contract=off, so the four operations round as written).
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
 
 __device__ __forceinline__ void fate_write(const Fate& f, int64_t id, int64_t step, int cause, double x, double y) {
     if (f.step) {
@@ -306,7 +319,8 @@ enum : int { S_FETCH = 0, S_TTS = 1, S_NI = 2, S_COLL = 3, S_SCAT = 4, S_END = 5
 // variant neither loads nor stores the per-phonon stream id and keeps that word a constant;
 // both variants draw the same numbers for such a model.
 // NT: threads per block (128, 256 or 512), a launch shape the autotuner chooses.
-template <int TALLY, bool MULTI, int NT>
+// FT: the flux tallies' element type, double or float (FP32 with Kahan-compensated sums).
+template <int TALLY, bool MULTI, int NT, typename FT>
 __global__ void __launch_bounds__(NT)
 __attribute__((amdgpu_waves_per_eu(TALLY == TALLY_NONE ? PSIM_UNTALLIED_WAVES_PER_EU : PSIM_TRANSPORT_WAVES_PER_EU)))
 transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const uint64_t* __restrict__ seed_p,
@@ -319,12 +333,12 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     // tallies cost six atomics whatever its length; rows 0..W-1 plus one for b = k1.
     const int WS = (TALLY == TALLY_LDS && tally.rec) ? tally.W * S : 0;
     const int WS1 = WS ? WS + S : 0;
-    double* lfx = reinterpret_cast<double*>(lds_raw);
-    double* lfy = lfx + WS1;
+    FT* lfx = reinterpret_cast<FT*>(lds_raw);
+    FT* lfy = lfx + WS1;
     int32_t* lcnt = reinterpret_cast<int32_t*>(lfy + WS1);
     for (int i = threadIdx.x; i < WS1; i += NT) {
-        lfx[i] = 0.;
-        lfy[i] = 0.;
+        lfx[i] = FT(0);
+        lfy[i] = FT(0);
         lcnt[i] = 0;
     }
     __syncthreads();
@@ -410,10 +424,10 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                     const int e0 = (a - k0 - 1) * S + cur_sensor, e1 = (in_last - k0) * S + cur_sensor;
                     atomicAdd(&lcnt[e0], p.sign);
                     atomicAdd(&lcnt[e1], -p.sign);
-                    atomicAdd(&lfx[e0], vx);
-                    atomicAdd(&lfx[e1], -vx);
-                    atomicAdd(&lfy[e0], vy);
-                    atomicAdd(&lfy[e1], -vy);
+                    atomicAdd(&lfx[e0], (FT)vx);
+                    atomicAdd(&lfx[e1], (FT)-vx);
+                    atomicAdd(&lfy[e0], (FT)vy);
+                    atomicAdd(&lfy[e1], (FT)-vy);
                     first = in_last + 1;
                 }
             }
@@ -422,10 +436,12 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
                 const int64_t t0 = row + first, t1 = row + last + 1;
                 atomicAdd(reinterpret_cast<unsigned long long*>(&tally.dcnt[t0]), (unsigned long long)(int64_t)p.sign);
                 atomicAdd(reinterpret_cast<unsigned long long*>(&tally.dcnt[t1]), (unsigned long long)(int64_t)-p.sign);
-                atomicAdd(&tally.dfx[t0], vx);
-                atomicAdd(&tally.dfx[t1], -vx);
-                atomicAdd(&tally.dfy[t0], vy);
-                atomicAdd(&tally.dfy[t1], -vy);
+                FT* dfx = static_cast<FT*>(tally.dfx);
+                FT* dfy = static_cast<FT*>(tally.dfy);
+                atomicAdd(&dfx[t0], (FT)vx);
+                atomicAdd(&dfx[t1], (FT)-vx);
+                atomicAdd(&dfy[t0], (FT)vy);
+                atomicAdd(&dfy[t1], (FT)-vy);
             }
             c.tallies += (uint32_t)(last - a + 1);
         }
@@ -746,16 +762,26 @@ transport_kernel(Model m, Pool pool, Lists lists, int32_t k0, int32_t k1, const 
     if (WS) {  // the difference arrays summed over the window's steps, per sensor, into the partials
         __syncthreads();
         const int64_t base = (int64_t)blockIdx.x * WS;
+        FT* pfx = static_cast<FT*>(tally.pfx);
+        FT* pfy = static_cast<FT*>(tally.pfy);
         for (int sn = threadIdx.x; sn < S; sn += NT) {
             int32_t cnt = 0;
-            double fx = 0., fy = 0.;
+            FT fx = FT(0), fy = FT(0);
+            float cx = 0.f, cy = 0.f;  // FP32: the running sums' compensations
             for (int e = sn; e < WS; e += S) {
                 cnt += lcnt[e];
-                fx += lfx[e];
-                fy += lfy[e];
+                if constexpr (std::is_same<FT, float>::value) {
+                    kahan_add(fx, cx, lfx[e]);
+                    kahan_add(fy, cy, lfy[e]);
+                    tally.pcx[base + e] = cx;
+                    tally.pcy[base + e] = cy;
+                } else {
+                    fx += lfx[e];
+                    fy += lfy[e];
+                }
                 tally.pcnt[base + e] = cnt;
-                tally.pfx[base + e] = fx;
-                tally.pfy[base + e] = fy;
+                pfx[base + e] = fx;
+                pfy[base + e] = fy;
             }
         }
     }
@@ -858,23 +884,33 @@ __global__ void __launch_bounds__(SORT_BLOCK) cell_scatter_kernel(Live src, Live
 }
 
 // Adds the HBM difference arrays [S][R + 1] of a run, summed along the recorded steps, into the
-// tallies [S][R] (one thread per sensor, steps in order).
+// tallies [S][R] (one thread per sensor, steps in order; FP32 arrays are summed with Kahan
+// compensation and added as sum - compensation).
+template <typename FT>
 __global__ void __launch_bounds__(BLOCK) diff_prefix_kernel(const int64_t* __restrict__ dcnt,
-    const double* __restrict__ dfx, const double* __restrict__ dfy, int S, int64_t R, int64_t* __restrict__ counts,
+    const FT* __restrict__ dfx, const FT* __restrict__ dfy, int S, int64_t R, int64_t* __restrict__ counts,
     double* __restrict__ flux) {
     const int s = blockIdx.x * BLOCK + threadIdx.x;
     if (s >= S) { return; }
     int64_t cnt = 0;
-    double fx = 0., fy = 0.;
+    FT fx = FT(0), fy = FT(0);
+    float cx = 0.f, cy = 0.f;
     for (int64_t r = 0; r < R; ++r) {
         const int64_t d = (int64_t)s * (R + 1) + r;
         cnt += dcnt[d];
-        fx += dfx[d];
-        fy += dfy[d];
         const int64_t t = (int64_t)s * R + r;
         counts[t] += cnt;
-        flux[2 * t] += fx;
-        flux[2 * t + 1] += fy;
+        if constexpr (std::is_same<FT, float>::value) {
+            kahan_add(fx, cx, dfx[d]);
+            kahan_add(fy, cy, dfy[d]);
+            flux[2 * t] += (double)fx - (double)cx;
+            flux[2 * t + 1] += (double)fy - (double)cy;
+        } else {
+            fx += dfx[d];
+            fy += dfy[d];
+            flux[2 * t] += fx;
+            flux[2 * t + 1] += fy;
+        }
     }
 }
 
@@ -896,9 +932,13 @@ __global__ void seed_kernel(uint64_t* __restrict__ seed_p, uint64_t seed) {
 // sensor) and adds its sums with atomics: enough blocks to fill the chip even for a window of a
 // few sensors.  Counts stay exact (integer atomics); flux sums in another order, as the LDS
 // atomics of the transport kernel already do.
+// FP32 partials (FT float) are summed with Kahan compensation, each block's partial entering as its
+// sum and its negated compensation, and added as sum - compensation.
+template <typename FT>
 __global__ void __launch_bounds__(BLOCK) reduce_kernel(const int32_t* __restrict__ pcnt,
-    const double* __restrict__ pfx, const double* __restrict__ pfy, int n_blocks, int per, int WS, int S, int64_t k0,
-    int64_t adj, int64_t R, int64_t* __restrict__ counts, double* __restrict__ flux) {
+    const FT* __restrict__ pfx, const FT* __restrict__ pfy, const float* __restrict__ pcx, const float* __restrict__ pcy,
+    int n_blocks, int per, int WS, int S, int64_t k0, int64_t adj, int64_t R, int64_t* __restrict__ counts,
+    double* __restrict__ flux) {
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     if (e >= WS) { return; }
     const int w = e / S, s = e % S;
@@ -907,11 +947,25 @@ __global__ void __launch_bounds__(BLOCK) reduce_kernel(const int32_t* __restrict
     const int g0 = blockIdx.y * per, g1 = min(n_blocks, g0 + per);
     int64_t cnt = 0;
     double fx = 0., fy = 0.;
-    for (int g = g0; g < g1; ++g) {
-        const int64_t k = (int64_t)g * WS + e;
-        cnt += pcnt[k];
-        fx += pfx[k];
-        fy += pfy[k];
+    if constexpr (std::is_same<FT, float>::value) {
+        float sx = 0.f, cx = 0.f, sy = 0.f, cy = 0.f;
+        for (int g = g0; g < g1; ++g) {
+            const int64_t k = (int64_t)g * WS + e;
+            cnt += pcnt[k];
+            kahan_add(sx, cx, pfx[k]);
+            kahan_add(sx, cx, -pcx[k]);
+            kahan_add(sy, cy, pfy[k]);
+            kahan_add(sy, cy, -pcy[k]);
+        }
+        fx = (double)sx - (double)cx;
+        fy = (double)sy - (double)cy;
+    } else {
+        for (int g = g0; g < g1; ++g) {
+            const int64_t k = (int64_t)g * WS + e;
+            cnt += pcnt[k];
+            fx += pfx[k];
+            fy += pfy[k];
+        }
     }
     const int64_t t = (int64_t)s * R + idx;
     if (cnt != 0) { atomicAdd(reinterpret_cast<unsigned long long*>(counts + t), (unsigned long long)cnt); }
@@ -954,8 +1008,10 @@ struct psim_gpu_ctx {
     double* d_dfy = nullptr;
     unsigned long long* d_counters = nullptr;
     int32_t* d_pcnt = nullptr;
-    double* d_pfx = nullptr;
+    double* d_pfx = nullptr;  // window partials; float in the FP32 tally precision
     double* d_pfy = nullptr;
+    float* d_pcx = nullptr;  // FP32 tally precision: Kahan compensations of the partials
+    float* d_pcy = nullptr;
     int64_t partial_cap = 0;
     bool fate_enabled = false;
     Fate fate{ nullptr, nullptr, nullptr };
@@ -975,6 +1031,7 @@ struct psim_gpu_ctx {
     int32_t window = 0;
     int32_t blocks_per_cu = 8;  // transport grid: n_CU x blocks_per_cu blocks
     int32_t block = 256;  // threads per transport block
+    bool tally_fp32 = false;  // flux tallies in FP32 with Kahan-compensated sums (else FP64)
     int32_t tally_mode = PSIM_TALLY_AUTO;  // where recorded steps are tallied
     // seed of the current run, read by the kernels from device memory (pinned host staging copy)
     uint64_t* d_seed = nullptr;
@@ -1068,9 +1125,13 @@ void free_partials(psim_gpu_ctx* c) {
     if (c->d_pcnt) { (void)hipFree(c->d_pcnt); }
     if (c->d_pfx) { (void)hipFree(c->d_pfx); }
     if (c->d_pfy) { (void)hipFree(c->d_pfy); }
+    if (c->d_pcx) { (void)hipFree(c->d_pcx); }
+    if (c->d_pcy) { (void)hipFree(c->d_pcy); }
     c->d_pcnt = nullptr;
     c->d_pfx = nullptr;
     c->d_pfy = nullptr;
+    c->d_pcx = nullptr;
+    c->d_pcy = nullptr;
     c->partial_cap = 0;
 }
 
@@ -1179,7 +1240,7 @@ int prepare_run(psim_gpu_ctx* c, RunPlan& plan) {
         c->gen++;
     }
     const int64_t blocks_needed = (n + BLOCK - 1) / BLOCK;  // emission and birth sort: one thread per phonon
-    const int tb = c->block;  // transport: c->block threads per block, n_CU x blocks_per_cu blocks at most
+    const int tb = c->tally_fp32 ? 256 : c->block;  // transport threads per block (FP32 tallies: 256)
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + tb - 1) / tb, (int64_t)c->n_cu * c->blocks_per_cu));
     // Each lane counts its phonon-steps, tallies and events in 32 bits until the launch ends (the
     // wave sums are 64-bit).  The work queue spreads the phonons evenly over the lanes, so a lane
@@ -1242,7 +1303,9 @@ int prepare_run(psim_gpu_ctx* c, RunPlan& plan) {
         c->d_pcnt = dalloc<int32_t>(cap);
         c->d_pfx = dalloc<double>(cap);
         c->d_pfy = dalloc<double>(cap);
-        if (!c->d_pcnt || !c->d_pfx || !c->d_pfy) {
+        c->d_pcx = dalloc<float>(cap);
+        c->d_pcy = dalloc<float>(cap);
+        if (!c->d_pcnt || !c->d_pfx || !c->d_pfy || !c->d_pcx || !c->d_pcy) {
             free_partials(c);
             return fail(c, PSIM_E_DEVICE, "hipMalloc failed for tally partials");
         }
@@ -1271,40 +1334,44 @@ int zero_fill(psim_gpu_ctx* c, void* p, size_t bytes) {
     return PSIM_OK;
 }
 
-// One transport launch of the variant for this tally path, stream layout and block size.
-template <int NT>
+// One transport launch of the variant for this tally path, stream layout, block size and flux type.
+template <int NT, typename FT>
 void launch_transport_nt(psim_gpu_ctx* c, int tally, int grid, size_t lds, const Lists& lists, int32_t k0, int32_t k1,
     const Tally& t, const Fate& fate) {
     const dim3 g((unsigned)grid), b(NT);
-#define PSIM_TRANSPORT(T, M)                                                                                 \
-    hipLaunchKernelGGL((transport_kernel<T, M, NT>), g, b, lds, c->stream, c->m, c->pool, lists, k0, k1, c->d_seed, t, \
+#define PSIM_TRANSPORT(T, M, F)                                                                              \
+    hipLaunchKernelGGL((transport_kernel<T, M, NT, F>), g, b, lds, c->stream, c->m, c->pool, lists, k0, k1, c->d_seed, t, \
         c->d_counters, fate)
+    // the untallied variant has no flux arrays: it is the same kernel for either flux type
     if (c->multi_stream) {
         if (tally == TALLY_NONE) {
-            PSIM_TRANSPORT(TALLY_NONE, true);
+            PSIM_TRANSPORT(TALLY_NONE, true, double);
         } else if (tally == TALLY_GLOBAL) {
-            PSIM_TRANSPORT(TALLY_GLOBAL, true);
+            PSIM_TRANSPORT(TALLY_GLOBAL, true, FT);
         } else {
-            PSIM_TRANSPORT(TALLY_LDS, true);
+            PSIM_TRANSPORT(TALLY_LDS, true, FT);
         }
     } else if (tally == TALLY_NONE) {
-        PSIM_TRANSPORT(TALLY_NONE, false);
+        PSIM_TRANSPORT(TALLY_NONE, false, double);
     } else if (tally == TALLY_GLOBAL) {
-        PSIM_TRANSPORT(TALLY_GLOBAL, false);
+        PSIM_TRANSPORT(TALLY_GLOBAL, false, FT);
     } else {
-        PSIM_TRANSPORT(TALLY_LDS, false);
+        PSIM_TRANSPORT(TALLY_LDS, false, FT);
     }
 #undef PSIM_TRANSPORT
 }
 
+// The FP32 tally precision runs 256-thread blocks only (one more set of kernels, not three).
 void launch_transport(psim_gpu_ctx* c, int tally, int grid, size_t lds, const Lists& lists, int32_t k0, int32_t k1,
     const Tally& t, const Fate& fate) {
-    if (c->block == 128) {
-        launch_transport_nt<128>(c, tally, grid, lds, lists, k0, k1, t, fate);
+    if (c->tally_fp32) {
+        launch_transport_nt<256, float>(c, tally, grid, lds, lists, k0, k1, t, fate);
+    } else if (c->block == 128) {
+        launch_transport_nt<128, double>(c, tally, grid, lds, lists, k0, k1, t, fate);
     } else if (c->block == 512) {
-        launch_transport_nt<512>(c, tally, grid, lds, lists, k0, k1, t, fate);
+        launch_transport_nt<512, double>(c, tally, grid, lds, lists, k0, k1, t, fate);
     } else {
-        launch_transport_nt<256>(c, tally, grid, lds, lists, k0, k1, t, fate);
+        launch_transport_nt<256, double>(c, tally, grid, lds, lists, k0, k1, t, fate);
     }
 }
 
@@ -1351,8 +1418,10 @@ int enqueue_run(psim_gpu_ctx* c, const RunPlan& plan, bool eager) {
             c->d_born_offset, w };
         // window steps k0+1 .. k1 map to tally indices (k - adj) in [0, R)
         const bool rec = (k1 >= P.step_adjustment) && (k0 + 1 - P.step_adjustment < R) && (k1 - k0 <= W);
-        Tally t{ c->d_pcnt, c->d_pfx, c->d_pfy, c->d_dcnt, c->d_dfx, c->d_dfy, rec ? 1 : 0, global_tally ? 1 : 0, W };
-        const size_t lds = (rec && !global_tally) ? (size_t)(WS + S) * 20 : 0;
+        Tally t{ c->d_pcnt, c->d_pfx, c->d_pfy, c->d_pcx, c->d_pcy, c->d_dcnt, c->d_dfx, c->d_dfy, rec ? 1 : 0,
+            global_tally ? 1 : 0, W };
+        // (W + 1) rows per sensor of a count and two flux values
+        const size_t lds = (rec && !global_tally) ? (size_t)(WS + S) * (c->tally_fp32 ? 12 : 20) : 0;
         if (eager) { HIP_CHECK(c, hipEventRecord(c->ev[1], c->stream)); }
         launch_transport(c, !rec ? TALLY_NONE : global_tally ? TALLY_GLOBAL : TALLY_LDS, grid, !rec ? 0 : lds, lists,
             (int32_t)k0, (int32_t)k1, t, fate);
@@ -1363,9 +1432,15 @@ int enqueue_run(psim_gpu_ctx* c, const RunPlan& plan, bool eager) {
             // split the block partials so that the reduce has ~4 blocks per CU
             const int splits = std::max(1, std::min(grid, (4 * c->n_cu + rblocks - 1) / rblocks));
             const int per = (grid + splits - 1) / splits;
-            hipLaunchKernelGGL(reduce_kernel, dim3(rblocks, (unsigned)((grid + per - 1) / per)), dim3(BLOCK), 0,
-                c->stream, c->d_pcnt, c->d_pfx, c->d_pfy, grid, per, (int)WS, S, k0, P.step_adjustment, R, c->d_counts,
-                c->d_flux);
+            const dim3 rg(rblocks, (unsigned)((grid + per - 1) / per));
+            if (c->tally_fp32) {
+                hipLaunchKernelGGL(reduce_kernel<float>, rg, dim3(BLOCK), 0, c->stream, c->d_pcnt,
+                    reinterpret_cast<const float*>(c->d_pfx), reinterpret_cast<const float*>(c->d_pfy), c->d_pcx, c->d_pcy,
+                    grid, per, (int)WS, S, k0, P.step_adjustment, R, c->d_counts, c->d_flux);
+            } else {
+                hipLaunchKernelGGL(reduce_kernel<double>, rg, dim3(BLOCK), 0, c->stream, c->d_pcnt, c->d_pfx, c->d_pfy,
+                    c->d_pcx, c->d_pcy, grid, per, (int)WS, S, k0, P.step_adjustment, R, c->d_counts, c->d_flux);
+            }
             HIP_CHECK(c, hipGetLastError());
         }
         if (w + 1 < (int32_t)n_windows) {  // survivors back into live[0], in cell order
@@ -1398,8 +1473,15 @@ int enqueue_run(psim_gpu_ctx* c, const RunPlan& plan, bool eager) {
         }
     }
     if (n > 0 && R > 0) {  // the segments tallied in HBM
-        hipLaunchKernelGGL(diff_prefix_kernel, dim3((unsigned)((S + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, c->stream,
-            c->d_dcnt, c->d_dfx, c->d_dfy, S, R, c->d_counts, c->d_flux);
+        const dim3 dg((unsigned)((S + BLOCK - 1) / BLOCK));
+        if (c->tally_fp32) {
+            hipLaunchKernelGGL(diff_prefix_kernel<float>, dg, dim3(BLOCK), 0, c->stream, c->d_dcnt,
+                reinterpret_cast<const float*>(c->d_dfx), reinterpret_cast<const float*>(c->d_dfy), S, R, c->d_counts,
+                c->d_flux);
+        } else {
+            hipLaunchKernelGGL(diff_prefix_kernel<double>, dg, dim3(BLOCK), 0, c->stream, c->d_dcnt, c->d_dfx, c->d_dfy,
+                S, R, c->d_counts, c->d_flux);
+        }
         HIP_CHECK(c, hipGetLastError());
     }
     return PSIM_OK;
@@ -1765,6 +1847,14 @@ int psim_gpu_set_block(psim_gpu_ctx* c, int32_t threads) {
     if (!c || (threads != 128 && threads != 256 && threads != 512)) { return PSIM_E_INVALID; }
     if (c->block != threads) { c->gen++; }
     c->block = threads;
+    return PSIM_OK;
+}
+
+int psim_gpu_set_tally_precision(psim_gpu_ctx* c, int32_t precision) {
+    if (!c || (precision != PSIM_TALLY_FP64 && precision != PSIM_TALLY_FP32_KAHAN)) { return PSIM_E_INVALID; }
+    const bool fp32 = precision == PSIM_TALLY_FP32_KAHAN;
+    if (c->tally_fp32 != fp32) { c->gen++; }
+    c->tally_fp32 = fp32;
     return PSIM_OK;
 }
 

@@ -187,3 +187,48 @@ def test_each_run_of_a_model_draws_from_its_own_seed(engine):
         np.testing.assert_array_equal(res.counts, o.counts)
         assert _flux_close(res.raw_flux, o.flux)
     assert not np.array_equal(model.results(0).counts, model.results(1).counts)
+
+
+# the FP32 tallies' bounds against FP64: every tally, relative to the run's largest, and the
+# per-sensor mean x-fluxes, relative to their largest (measured on an MI355X: 7.0e-7 and 2.0e-6 on
+# the LDS path, whose partials are Kahan sums; 7.4e-6 and 8.9e-6 on the HBM path, where every
+# recorded segment is an uncompensated FP32 atomic into the run-long difference arrays)
+FP32_CASES = {
+    "lshape_lds": (configs.lshape_full(num_phonons=200_000, num_measurements=1000), "lds", 2e-6, 1e-5),
+    "bar_400_sensors_hbm": (configs.linear_bar(num_sensors=400, length=40000.0, num_phonons=100_000,
+                                               num_measurements=1000, sim_time=5.0, spec=0.4, sim_type=1,
+                                               step_interval=50).to_dict(), "hbm", 2e-5, 5e-5),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FP32_CASES))
+def test_fp32_kahan_tallies_against_fp64(engine, name):
+    """The FP32 tally precision (FP32 atomics into the difference arrays, Kahan-compensated FP32
+    sums along the steps and across blocks) runs the FP64 run's trajectories on the same seed:
+    counts and counters are equal, every flux tally lies within the case's bound of the FP64 run
+    (relative to its largest tally), and so do the per-sensor mean x-fluxes (relative to their
+    largest): far inside the statistical spread of a run, about 1% per sensor mean."""
+    cfg, tally, bound, mean_bound = FP32_CASES[name]
+    print(f"case {name} fp32 kahan, tally {tally}", flush=True)
+    model = psim_amd.Model(cfg)
+    model.upload(engine, 0)
+    engine.set_window(0)
+    engine.set_tally_mode(tally)
+    try:
+        engine.run_resident(11)
+        c64, f64, n64 = engine.fetch(model.num_sensors, model.num_record)
+        engine.set_tally_precision("fp32_kahan")
+        engine.run_resident(11)
+        c32, f32, n32 = engine.fetch(model.num_sensors, model.num_record)
+    finally:
+        engine.set_tally_precision("fp64")
+        engine.set_tally_mode("auto")
+    np.testing.assert_array_equal(c32, c64)
+    assert n32 == n64
+    scale = float(np.max(np.abs(f64)))
+    err = float(np.max(np.abs(f32 - f64)))
+    m32, m64 = f32[:, :, 0].mean(axis=1), f64[:, :, 0].mean(axis=1)
+    mean_err = float(np.max(np.abs(m32 - m64)) / np.max(np.abs(m64)))
+    print(f"max |flux32 - flux64| / max |flux64| = {err / scale:.3e}; sensor mean x-flux {mean_err:.3e}", flush=True)
+    assert err <= bound * scale
+    assert mean_err <= mean_bound
