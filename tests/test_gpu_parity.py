@@ -232,3 +232,17 @@ def test_fp32_kahan_tallies_against_fp64(engine, name):
     print(f"max |flux32 - flux64| / max |flux64| = {err / scale:.3e}; sensor mean x-flux {mean_err:.3e}", flush=True)
     assert err <= bound * scale
     assert mean_err <= mean_bound
+
+
+def test_launch_shape_setters_reject_bad_values(engine):
+    """Block sizes other than 128/256/512 threads and unknown tally precisions are refused with an
+    error code; the launch shape in use is unchanged by a refused call."""
+    L = psim_amd.lib()
+    before = engine.launch_shape()
+    assert L.psim_gpu_set_block(engine.handle, 100) != 0
+    assert L.psim_gpu_set_block(engine.handle, 1024) != 0
+    assert L.psim_gpu_set_tally_precision(engine.handle, 2) != 0
+    assert engine.launch_shape() == before
+    engine.set_block(512)
+    assert engine.launch_shape()["block"] == 512
+    engine.set_block(256)
