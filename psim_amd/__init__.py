@@ -23,7 +23,8 @@ import numpy as np
 from . import configs  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpsim_amd.so")
+# PSIM_AMD_LIB names another build of the same sources (the AddressSanitizer build of csrc/asan.mk)
+LIB_PATH = os.environ.get("PSIM_AMD_LIB") or os.path.join(_HERE, "libpsim_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "psim_gpu.h")
 
 COUNTER_NAMES = ["phonon_steps", "scatters", "impacts", "relocations", "absorbed", "expired",

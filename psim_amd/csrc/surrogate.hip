@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <array>
+#include <climits>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -679,8 +680,11 @@ bool aligned16(const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) ==
 // outnumber the counters, the product runs unsplit.  Returns the tile index used.
 template <int EPI>
 int launch_nt(int device, NtArgs a, hipStream_t st, bool may_tune, size_t ws_elems) {
-    a.vec_a = aligned16(a.A) && a.K % 8 == 0;
-    a.vec_b = aligned16(a.B) && a.K % 8 == 0;
+    // buffer loads address a block's rows with 32-bit byte offsets: rows x K x 2 < 2^31 for the
+    // tallest tile (256 rows), else element loads
+    const bool fits = (int64_t)256 * a.K * 2 < (int64_t)INT32_MAX;
+    a.vec_a = fits && aligned16(a.A) && a.K % 8 == 0;
+    a.vec_b = fits && aligned16(a.B) && a.K % 8 == 0;
     a.vec_out = a.N % 8 == 0 && aligned16(a.c) && aligned16(a.ws) && aligned16(a.out) && aligned16(a.mask);
     a.vec_out_t = a.M % 8 == 0 && aligned16(a.out_t);
     int splits = EPI == EPI_F32 && a.ws && a.counters ? nt_splits(a.M, a.N, a.K) : 1;
@@ -1394,7 +1398,9 @@ int psim_mlp_time_product(psim_mlp* h, int32_t kind, int32_t rows, int32_t reps,
 // lane layout and the transposed loads can be checked against a host product.
 int psim_mlp_gemm_check(int device, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const float* A,
     const float* B, float* C) {
-    if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0 || M > MAX_ROWS) { return PSIM_E_INVALID; }
+    if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0 || M > MAX_ROWS || N > MAX_ROWS || K > MAX_ROWS) {
+        return PSIM_E_INVALID;
+    }
     if (hipSetDevice(device) != hipSuccess) { return PSIM_E_DEVICE; }
     const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
     std::vector<uint16_t> ha(na), hb(nb);
@@ -1432,7 +1438,9 @@ int psim_mlp_gemm_check(int device, int32_t ta, int32_t tb, int32_t M, int32_t N
 // stream of its own (operands filled on the device).
 int psim_mlp_time_gemm(int device, int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, int32_t reps,
     float* ms_per_launch) {
-    if (!ms_per_launch || M <= 0 || N <= 0 || K <= 0 || reps <= 0 || M > MAX_ROWS) { return PSIM_E_INVALID; }
+    if (!ms_per_launch || M <= 0 || N <= 0 || K <= 0 || reps <= 0 || M > MAX_ROWS || N > MAX_ROWS || K > MAX_ROWS) {
+        return PSIM_E_INVALID;
+    }
     if (hipSetDevice(device) != hipSuccess) { return PSIM_E_DEVICE; }
     const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
     uint16_t *da = nullptr, *db = nullptr;
