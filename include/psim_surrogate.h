@@ -50,8 +50,15 @@ int psim_mlp_forward_noalloc(psim_mlp* mlp, const float* x, int32_t rows, float*
 /* Mean squared error against t and its gradient with respect to every parameter (FP32). */
 int psim_mlp_gradients(psim_mlp* mlp, const float* x, const float* t, int32_t rows, float* grad, float* loss);
 
-/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the mean squared error of one batch. */
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the mean squared error of one batch.  The first
+ * step at a batch size runs eagerly (and tunes the products' tiles); the next one captures the
+ * step's kernels (forward, backward, Adam) as a HIP graph, which later steps of that size replay;
+ * the batch's copy to the device and the loss read-back stay outside the graph.  Replayed and
+ * eager steps give the same parameters bit for bit. */
 int psim_mlp_train_step(psim_mlp* mlp, const float* x, const float* t, int32_t rows, float lr, float* loss);
+
+/* Training steps replayed from a graph (1, the default) or all eager (0). */
+int psim_mlp_set_graph(psim_mlp* mlp, int32_t enable);
 
 /* C[M][N] = op(A)[M][K] x op(B)[K][N] on the MFMA GEMM used by the network, with the operands
  * rounded to bf16 (A stored [K][M] when ta, B stored [N][K] when tb): a check of the kernel. */

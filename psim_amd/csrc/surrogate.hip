@@ -790,12 +790,15 @@ __device__ inline void store_bf16_copies(const LayerTable& lt, int64_t i, uint16
     }
 }
 
-// Adam on the FP32 master parameters, then their bf16 copies (masked entries stay 0).
+// Adam on the FP32 master parameters, then their bf16 copies (masked entries stay 0).  The step's
+// learning rate and bias corrections (lr, 1 - b1^t, 1 - b2^t) are read from device memory, so a
+// captured step replays with the values copied in before each launch.
 __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
     float* __restrict__ m2, const float* __restrict__ mask, uint16_t* __restrict__ pb, uint16_t* __restrict__ pbT,
-    LayerTable lt, int64_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    LayerTable lt, int64_t n, const float* __restrict__ step_consts, float b1, float b2, float eps) {
     const int64_t i = (int64_t)blockIdx.x * EW_BLOCK + threadIdx.x;
     if (i >= n) { return; }
+    const float lr = step_consts[0], c1 = step_consts[1], c2 = step_consts[2];
     const float gi = g[i] * (mask ? mask[i] : 1.f);
     const float a = b1 * m1[i] + (1.f - b1) * gi;
     const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
@@ -864,6 +867,17 @@ struct psim_mlp {
     uint16_t *dzT = nullptr, *dz2T = nullptr;  // the same transposed
     // device address of this GPU's `scratch` array (psim_mlp_forward_noalloc), looked up at create
     unsigned char* scratch = nullptr;
+    // training steps replayed from a HIP graph: the step's kernels (forward, backward, Adam) captured
+    // once per batch size after one eager step at that size has tuned the products' tiles
+    bool use_graph = true;
+    float* step_consts = nullptr;  // device: lr, 1 - b1^t, 1 - b2^t of the current step
+    float* h_step_consts = nullptr;  // pinned host staging of the same
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_rows = 0;  // batch size of the captured step (0: none)
+    int64_t graph_gen = 0;  // alloc_gen when it was captured
+    int warm_rows = 0;  // batch size of the last eager step
+    int64_t alloc_gen = 0;  // bumped when batch or training buffers move
 };
 
 namespace {
@@ -931,6 +945,7 @@ int ensure_batch(psim_mlp* h, int rows) {
         return mlp_fail(h, PSIM_E_DEVICE, "hipMalloc failed for the surrogate batch buffers");
     }
     h->cap = cap;
+    h->alloc_gen++;
     return PSIM_OK;
 }
 
@@ -954,6 +969,7 @@ int ensure_train(psim_mlp* h, int rows) {
         return mlp_fail(h, PSIM_E_DEVICE, "hipMalloc failed for the surrogate training buffers");
     }
     h->train_cap = cap;
+    h->alloc_gen++;
     return PSIM_OK;
 }
 
@@ -1168,7 +1184,12 @@ int psim_mlp_create(psim_mlp** out, int device, int32_t n_in, int32_t hidden, in
     h->ws_elems = (size_t)MAX_SPLITS * widest;
     h->ws = mlp_alloc<float>(h->ws_elems);
     h->counters = mlp_alloc<unsigned>(NT_COUNTERS);
-    if (!h->p || !h->g || !h->m1 || !h->m2 || !h->mask || !h->pb || !h->pbT || !h->ws || !h->counters) {
+    h->step_consts = mlp_alloc<float>(4);
+    if (hipHostMalloc(reinterpret_cast<void**>(&h->h_step_consts), 4 * sizeof(float), hipHostMallocDefault) != hipSuccess) {
+        h->h_step_consts = nullptr;
+    }
+    if (!h->p || !h->g || !h->m1 || !h->m2 || !h->mask || !h->pb || !h->pbT || !h->ws || !h->counters || !h->step_consts
+        || !h->h_step_consts) {
         psim_mlp_destroy(h);
         return PSIM_E_DEVICE;
     }
@@ -1201,9 +1222,12 @@ void psim_mlp_destroy(psim_mlp* h) {
     if (!h) { return; }
     (void)hipSetDevice(h->device);
     if (h->stream) { (void)hipStreamSynchronize(h->stream); }
+    if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); }
+    if (h->graph) { (void)hipGraphDestroy(h->graph); }
+    if (h->h_step_consts) { (void)hipHostFree(h->h_step_consts); }
     free_train(h);
     free_batch(h);
-    void* ps[] = { h->p, h->g, h->m1, h->m2, h->mask, h->pb, h->pbT, h->ws, h->counters };
+    void* ps[] = { h->p, h->g, h->m1, h->m2, h->mask, h->pb, h->pbT, h->ws, h->counters, h->step_consts };
     for (void* q : ps) {
         if (q) { (void)hipFree(q); }
     }
@@ -1314,16 +1338,59 @@ int psim_mlp_train_step(psim_mlp* h, const float* x, const float* t, int32_t row
     MLP_CHECK(h, hipSetDevice(h->device));
     int rc;
     if ((rc = load_batch(h, x, t, rows)) != 0) { return rc; }
-    if ((rc = forward(h, rows, true)) != 0) { return rc; }
-    if ((rc = backward(h, rows)) != 0) { return rc; }
     ++h->step;
     const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.f - std::pow(b1, (float)h->step), c2 = 1.f - std::pow(b2, (float)h->step);
-    hipLaunchKernelGGL(adam_kernel, dim3(ew_blocks(h->n_params)), dim3(EW_BLOCK), 0, h->stream, h->p, h->g, h->m1, h->m2,
-        h->mask, h->pb, h->pbT, layer_table(h), h->n_params, lr, b1, b2, 1e-8f, c1, c2);
-    MLP_CHECK(h, hipGetLastError());
+    h->h_step_consts[0] = lr;
+    h->h_step_consts[1] = 1.f - std::pow(b1, (float)h->step);
+    h->h_step_consts[2] = 1.f - std::pow(b2, (float)h->step);
+    MLP_CHECK(h, hipMemcpyAsync(h->step_consts, h->h_step_consts, 3 * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    // The step's kernels: forward, backward, Adam (all on h->stream, no host synchronisation).
+    const auto enqueue = [&]() -> int {
+        int r;
+        if ((r = forward(h, rows, true)) != 0) { return r; }
+        if ((r = backward(h, rows)) != 0) { return r; }
+        hipLaunchKernelGGL(adam_kernel, dim3(ew_blocks(h->n_params)), dim3(EW_BLOCK), 0, h->stream, h->p, h->g, h->m1,
+            h->m2, h->mask, h->pb, h->pbT, layer_table(h), h->n_params, h->step_consts, b1, b2, 1e-8f);
+        MLP_CHECK(h, hipGetLastError());
+        return PSIM_OK;
+    };
+    if (h->graph_rows && (h->alloc_gen != h->graph_gen || h->graph_rows != rows)) {  // buffers moved or another size
+        if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); }
+        if (h->graph) { (void)hipGraphDestroy(h->graph); }
+        h->graph_exec = nullptr;
+        h->graph = nullptr;
+        h->graph_rows = 0;
+    }
+    if (!h->use_graph || (h->graph_rows != rows && h->warm_rows != rows)) {
+        // eager: the first step at this batch size (its products' tiles are tuned here)
+        if ((rc = enqueue()) != 0) { return rc; }
+        h->warm_rows = rows;
+    } else {
+        if (h->graph_rows != rows) {  // capture the step once, then replay it
+            MLP_CHECK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue();
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(h->stream, &g);
+            if (rc != 0) {
+                if (g) { (void)hipGraphDestroy(g); }
+                return rc;
+            }
+            if (e != hipSuccess) { return mlp_fail(h, PSIM_E_DEVICE, std::string("capturing a training step: ") + hipGetErrorString(e)); }
+            h->graph = g;
+            MLP_CHECK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+            h->graph_rows = rows;
+            h->graph_gen = h->alloc_gen;
+        }
+        MLP_CHECK(h, hipGraphLaunch(h->graph_exec, h->stream));
+    }
     if (loss) { return read_loss(h, rows, loss); }
     MLP_CHECK(h, hipStreamSynchronize(h->stream));
+    return PSIM_OK;
+}
+
+int psim_mlp_set_graph(psim_mlp* h, int32_t enable) {
+    if (!h) { return PSIM_E_INVALID; }
+    h->use_graph = enable != 0;
     return PSIM_OK;
 }
 

@@ -53,6 +53,7 @@ def _bind():
                                               ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _F]),
         "psim_mlp_time_product": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _F,
                                                  ctypes.POINTER(ctypes.c_int32)]),
+        "psim_mlp_set_graph": (ctypes.c_int, [_VP, ctypes.c_int32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -165,6 +166,10 @@ class Surrogate:
         loss = ctypes.c_float()
         self._check(self._L.psim_mlp_gradients(self._h, _fp(x), _fp(t), x.shape[0], _fp(g), ctypes.byref(loss)))
         return loss.value, g
+
+    def set_graph(self, enable: bool) -> None:
+        """Training steps replayed from a captured HIP graph (default) or all eager."""
+        self._check(self._L.psim_mlp_set_graph(self._h, int(bool(enable))))
 
     PRODUCTS = {"forward": 0, "forward_train": 1, "weight_grad": 2, "input_grad": 3}
 

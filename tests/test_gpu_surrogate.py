@@ -192,3 +192,26 @@ def test_every_tile_and_split_sum_gives_the_same_bits(surrogate, monkeypatch):
         np.testing.assert_array_equal(split, results[0][1])
         np.testing.assert_array_equal(y, results[0][2])
         np.testing.assert_array_equal(grad, results[0][3])
+
+
+def test_graph_replayed_training_steps_equal_eager_steps(surrogate):
+    """Training steps replayed from the captured graph (the default: eager at a new batch size,
+    captured on the next step, replayed after) give the same losses and parameters bit for bit as
+    all-eager steps, with eager work (a forward pass) between the steps and a change of batch
+    size that makes the step be captured again."""
+    rng = np.random.default_rng(17)
+    graphed = surrogate.Surrogate(seed=6)
+    eager = surrogate.Surrogate(seed=6)
+    eager.set_graph(False)
+    probe = rng.uniform(-1, 1, size=(300, surrogate.N_FEATURES)).astype(np.float32)
+    for k, rows in enumerate([1024, 1024, 1024, 1024, 2048, 2048, 2048, 1024]):
+        print(f"training step {k}, {rows} rows", flush=True)
+        x = rng.uniform(-1, 1, size=(rows, surrogate.N_FEATURES)).astype(np.float32)
+        t = rng.uniform(-1, 1, size=rows).astype(np.float32)
+        lg = graphed.train_step(x, t, 1e-3)
+        le = eager.train_step(x, t, 1e-3)
+        assert lg == le
+        np.testing.assert_array_equal(graphed.params(), eager.params())
+        np.testing.assert_array_equal(graphed.predict(probe), eager.predict(probe))
+    graphed.close()
+    eager.close()

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 20
     surrogate_names = psim_amd.exported_symbols_of_header(os.path.join(os.path.dirname(psim_amd.HEADER_PATH),
                                                                        "psim_surrogate.h"))
-    assert len(surrogate_names) == 13  # 11, psim_mlp_forward_noalloc and psim_mlp_time_product
+    assert len(surrogate_names) == 14  # 11, psim_mlp_forward_noalloc, psim_mlp_time_product, psim_mlp_set_graph
     names += surrogate_names
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing
