@@ -121,6 +121,28 @@ def test_batch_equals_single_runs(engine):
     assert batch_counters == total
 
 
+@pytest.mark.parametrize("block", [128, 512])
+def test_batch_block_size_does_not_change_results(engine, block):
+    """The batched (multi-stream) transport variant at 128 and 512 threads per block gives the
+    256-thread run's counters and every model's integer tallies exactly, its flux sums to 1e-9."""
+    cfgs = _small_batch(6)
+    print(f"batch of {len(cfgs)} geometries, block {block}", flush=True)
+    base_models = [psim_amd.Model(c) for c in cfgs]
+    engine.set_block(256)
+    base = psim_amd.run_batch(engine, base_models, seed=4)
+    models = [psim_amd.Model(c) for c in cfgs]
+    engine.set_block(block)
+    try:
+        got = psim_amd.run_batch(engine, models, seed=4)
+    finally:
+        engine.set_block(256)
+    assert got == base
+    for bm, m in zip(base_models, models):
+        a, b = bm.results(0), m.results(0)
+        np.testing.assert_array_equal(a.counts, b.counts)
+        assert _flux_close(b.raw_flux, a.raw_flux)
+
+
 def test_full_batch_of_256_structures(engine):
     """The benchmark's batch at its full size (256 structures of configs.batch_geometries(seed 0),
     1e4-1e6 phonons each, 2000 steps of 5 ps): the batch counters are the sums of the 256 single

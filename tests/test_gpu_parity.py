@@ -198,6 +198,8 @@ FP32_CASES = {
     "bar_400_sensors_hbm": (configs.linear_bar(num_sensors=400, length=40000.0, num_phonons=100_000,
                                                num_measurements=1000, sim_time=5.0, spec=0.4, sim_type=1,
                                                step_interval=50).to_dict(), "hbm", 2e-5, 5e-5),
+    "film_transient_lds": (configs.film_transient(num_phonons=200_000, num_measurements=2000, sim_time=2.0), "lds",
+                           2e-6, 1e-5),
 }
 
 
