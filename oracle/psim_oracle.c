@@ -1177,6 +1177,7 @@ static double impact_time_axis(double start, double end, double velocity, double
 /* ModelSimulator::nextImpact, modelSimulator.cpp:87-122; returns 1 and the impact time on a hit */
 static int next_impact(const or_model* m, phonon_t* p, double time, double* out_time, psim_stream* st,
     tally_t* ta) {
+    (void)st;  /* nextImpact draws nothing */
     const double vx = p->dx * p->velocity;
     const double vy = p->dy * p->velocity;
     const pt start = { p->px, p->py };
