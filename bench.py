@@ -370,10 +370,15 @@ def batch_bench(args) -> dict:
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
-# HBM bytes of one 65536 x 512 x 512 fused forward product (128 x 128 tile, 8 waves), copied from the
-# counter profile profiles/gemm_nt_kernel_surrogate.md: FETCH_SIZE 39469 KiB x 1.9911 + WRITE_SIZE
-# 83280 KiB x 0.9662 (copy-kernel calibration) per dispatch; counters cannot be read in this process
-GEMM_FWD_65536_TRAFFIC_BYTES = 162.9e6
+# HBM bytes of one 65536 x 512 x 512 fused forward product, per kernel the autotuner may pick for it,
+# copied from the counter profiles (FETCH_SIZE x 1.9910 + WRITE_SIZE x 0.9666, the copy-kernel
+# calibration, per dispatch; counters cannot be read in this process):
+# - weights resident in LDS, 8 waves: 35001 KiB and 90112 KiB (profiles/gemm_nt_bres_kernel_surrogate.md);
+# - the 128 x 128 tile: 39469 KiB and 83280 KiB (profiles/gemm_nt_kernel_surrogate.md).
+GEMM_FWD_65536_TRAFFIC = {
+    "512x128 (weights resident in LDS)": (160.5e6, "gemm_nt_bres_kernel", "profiles/gemm_nt_bres_kernel_surrogate.md"),
+    "128x128": (162.9e6, "gemm_nt_kernel", "profiles/gemm_nt_kernel_surrogate.md"),
+}
 
 
 def hipblaslt_gemm(M: int, N: int, K: int) -> dict:
@@ -435,18 +440,22 @@ def surrogate_products(rows: int = 0) -> tuple:
 
 def gemm_roofline(gemm: dict) -> dict:
     dom = gemm["forward_hidden_65536_rows"]
-    return {"bound": "mfma", "kernel": "gemm_nt_kernel",
+    # the kernel the autotuner picked; traffic only for the kernels whose counters were measured
+    traffic, kernel, source = GEMM_FWD_65536_TRAFFIC.get(
+        dom["tile"], (None, "gemm_nt_bres_kernel" if "resident" in dom["tile"] else "gemm_nt_kernel", None))
+    return {"bound": "mfma", "kernel": kernel,
             "shape": "65536 x 512 x 512 (forward of a hidden layer, bias + ReLU + bf16 out fused)",
             "achieved": dom["tflops"], "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": dom["tflops"] / MFMA_BF16_PEAK_TFLOPS, "tile": dom["tile"],
             # HBM bytes per launch (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration)
-            "traffic": GEMM_FWD_65536_TRAFFIC_BYTES,
-            "traffic_copied_from": "profiles/gemm_nt_kernel_surrogate.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
+            "traffic": traffic,
+            "traffic_copied_from": f"{source} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)" if source else None,
             "algorithmic_bytes": 2.0 * (65536 * 512 + 512 * 512 + 65536 * 512)}
 
 
 def gemm_bench(args) -> dict:
-    """The surrogate's MFMA products alone (no Monte Carlo): what the gemm_nt_kernel profile measures."""
+    """The surrogate's MFMA products alone (no Monte Carlo): what the gemm_nt_kernel and
+    gemm_nt_bres_kernel profiles measure."""
     import torch  # noqa: F401  (before the library initialises HIP)
 
     t0 = time.perf_counter()

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <array>
+#include <atomic>
 #include <climits>
 #include <cstring>
 #include <map>
@@ -38,13 +39,28 @@ constexpr int EW_BLOCK = 256;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// float -> bfloat16 bits, round to nearest even (NaN kept quiet).
+// float -> bfloat16 bits, round to nearest even (NaN kept quiet).  On the device this is one
+// v_cvt_pk_bf16_f32 (round to nearest even in the default mode, denormals kept), which gives the
+// host routine's bits for every finite value; the host routine costs about seven VALU operations.
 __host__ __device__ inline uint16_t f32_to_bf16(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
+#else
     uint32_t u;
     std::memcpy(&u, &f, 4);
     if ((u & 0x7fffffffu) > 0x7f800000u) { return (uint16_t)((u >> 16) | 0x40u); }
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
+#endif
+}
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// Two floats -> two bf16 in one word (lo in the low half), as f32_to_bf16 rounds each: one
+// v_cvt_pk_bf16_f32 for the pair.
+__device__ inline uint32_t pack_bf16x2(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{ lo, hi }, bf16x2));
 }
 
 __host__ __device__ inline float bf16_to_f32(uint16_t h) {
@@ -68,6 +84,7 @@ constexpr int BK = 32;
 constexpr int LDK = BK + 8;  // LDS row pitch in bf16 (80 bytes: 16-byte aligned, staggers banks)
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 template <bool TA, bool TB, int BM, int BN, int WM, int WN>
 __global__ void __launch_bounds__(GEMM_THREADS) gemm_bf16_kernel(const uint16_t* __restrict__ A,
@@ -480,23 +497,25 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC) gemm_nt_kernel(NtArgs p) 
                 float b[8];
                 std::memcpy(b, p.bias + n, 16);
                 std::memcpy(b + 4, p.bias + n + 4, 16);
-                for (int j = 0; j < 8; ++j) {
-                    v[j] += b[j];
-                    o[j] = f32_to_bf16(p.relu ? fmaxf(v[j], 0.f) : v[j]);
-                }
+                for (int j = 0; j < 8; ++j) { v[j] += b[j]; }
                 if (p.c) {
                     std::memcpy(p.c + at, v, 16);
                     std::memcpy(p.c + at + 4, v + 4, 16);
                 }
+                // ReLU in place: the transposed pass below applies it again, to the same effect
+                if (p.relu) {
+                    for (int j = 0; j < 8; ++j) { v[j] = fmaxf(v[j], 0.f); }
+                }
             } else {
                 uint16_t mk[8];
                 std::memcpy(mk, p.mask + at, 16);
-                for (int j = 0; j < 8; ++j) {
-                    v[j] = bf16_to_f32(mk[j]) > 0.f ? v[j] : 0.f;
-                    o[j] = f32_to_bf16(v[j]);
-                }
+                for (int j = 0; j < 8; ++j) { v[j] = bf16_to_f32(mk[j]) > 0.f ? v[j] : 0.f; }
             }
-            if (p.out) { std::memcpy(p.out + at, o, 16); }
+            if (p.out) {
+                const u32x4 w = { pack_bf16x2(v[0], v[1]), pack_bf16x2(v[2], v[3]), pack_bf16x2(v[4], v[5]),
+                    pack_bf16x2(v[6], v[7]) };
+                *reinterpret_cast<u32x4*>(p.out + at) = w;
+            }
         } else {
             const int nn = min(8, p.N - n);
             for (int j = 0; j < 8; ++j) {
@@ -508,7 +527,9 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC) gemm_nt_kernel(NtArgs p) 
                     o[j] = f32_to_bf16(v[j]);
                 }
             }
-            for (int j = 0; j < nn; ++j) {
+            #pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (j >= nn) { continue; }
                 if (EPI == EPI_FWD && p.c) { p.c[at + j] = v[j]; }
                 if (p.out) { p.out[at + j] = o[j]; }
             }
@@ -524,17 +545,206 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC) gemm_nt_kernel(NtArgs p) 
     for (int c = tid; c < BN * RCH; c += NT_THREADS) {
         const int col = c % BN, r8 = (c / BN) * 8, n = n0 + col, m = m0 + r8;
         if (n >= p.N || m >= p.M) { continue; }
-        uint16_t o[8];
+        float y[8];
         for (int j = 0; j < 8; ++j) {
             const float x = Cs[r8 + j][col];
-            o[j] = f32_to_bf16(EPI == EPI_FWD && p.relu ? fmaxf(x, 0.f) : x);
+            y[j] = EPI == EPI_FWD && p.relu ? fmaxf(x, 0.f) : x;
         }
+        const u32x4 w = { pack_bf16x2(y[0], y[1]), pack_bf16x2(y[2], y[3]), pack_bf16x2(y[4], y[5]),
+            pack_bf16x2(y[6], y[7]) };
         uint16_t* dst = p.out_t + (int64_t)n * p.M + m;
         if (p.vec_out_t && m + 8 <= p.M) {
-            std::memcpy(dst, o, 16);
+            *reinterpret_cast<u32x4*>(dst) = w;
         } else {
-            for (int j = 0; j < 8 && m + j < p.M; ++j) { dst[j] = o[j]; }
+            #pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (m + j < p.M) { dst[j] = (uint16_t)(w[j / 2] >> (16 * (j % 2))); }
+            }
         }
+    }
+}
+
+// K-major products with the weight slice resident in LDS (N a multiple of 128, K a multiple of 128
+// or 256 up to 512: the network's forward and input-gradient products).  The tiles above stage
+// both operands through LDS at every K step, and at 64 x 32 per wave they read about one 16-byte
+// LDS vector per MFMA.  Here a block loads rows n0 .. n0 + 127 of B (all of K, 128 KB) into LDS
+// once and keeps it: each of its 4 or 8 waves computes a 64 x 128 tile (32 accumulators) of a band
+// of 64 rows per wave, reads its A fragments straight from global memory into a register ring
+// (chunks of 32 K ahead), and its B fragments from LDS (one vector per 4 MFMAs); no barrier after
+// the weights are in.  Blocks are persistent over bands: a block keeps one column slice and steps
+// through the bands dealt to it, the slices of one band on the same XCD, so A is fetched from HBM
+// once and shared in L2.  Each output element sums the same 32-deep chunks, in the same order and
+// the same lane layout, as in gemm_nt_kernel; the MFMA takes B as its first operand, so a lane
+// holds 4 consecutive columns of one row (16-byte FP32, 8-byte bf16 row stores).
+constexpr int BR_BN = 128;  // columns of the resident weight slice
+constexpr int BR_WROWS = 64;  // rows per wave and band
+constexpr int BR_KMAX = 512;
+constexpr int BR_PITCH = BR_KMAX / 8;  // LDS row pitch in 16-byte vectors (1024 bytes), slots swizzled
+
+// WAVES 4: one wave per SIMD (512 registers), eight chunks of A ahead, B fragments read a chunk
+// ahead.  WAVES 8: two waves per SIMD (256 registers each), four chunks ahead, each wave's waits
+// covered by the other's MFMAs.  A band is WAVES x 64 rows.
+template <int WAVES>
+struct BresShape {
+    static constexpr int band = WAVES * BR_WROWS;
+    static constexpr int ahead = WAVES == 4 ? 8 : 4;  // 32-deep chunks of A in flight per wave
+    static constexpr bool b_ahead = WAVES == 4;  // B fragments double-buffered
+};
+
+template <int EPI, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES, WAVES / 4) gemm_nt_bres_kernel(NtArgs p, int groups) {
+    constexpr int BR_BAND = BresShape<WAVES>::band, BR_AHEAD = BresShape<WAVES>::ahead, THREADS = 64 * WAVES;
+    __shared__ u32x4 Bs[BR_BN][BR_PITCH];
+    // the wave index is uniform: read it as a scalar, so the A descriptors built from it are too
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid / 64), lane = tid % 64, kq = lane / 16;
+    // block -> (column slice, band group): the host launches XCDS x slices x groups blocks; blocks
+    // are dealt to the XCDs in turn, and the slices of a band group sit on one XCD
+    const int slices = p.N / BR_BN;
+    const int x = (int)blockIdx.x % XCDS, q = (int)blockIdx.x / XCDS;
+    const int n0 = (q % slices) * BR_BN;
+    const int gi = x * groups + q / slices, stride = XCDS * groups;
+    const int bands = (p.M + BR_BAND - 1) / BR_BAND;
+    if (gi >= bands) { return; }  // the whole block
+    const int my_bands = (bands - gi + stride - 1) / stride;
+    const int nc = p.K / 32;  // chunks per band (a multiple of BR_AHEAD)
+
+    // A fragments of a chunk: lane holds row (i * 16 + lane % 16) of the wave's 64, K columns
+    // chunk * 32 + kq * 8 .. + 8.  One descriptor over all of A (M K 2 < 2^31 - 2^20, so no byte
+    // offset wraps): rows past M, and every row of a chunk past the block's last band (row base M),
+    // lie past its records and load zeros, with no branch around the loads.
+    u32x4 ra[BR_AHEAD][4];
+    const __amdgpu_buffer_rsrc_t rs_a = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint16_t*>(p.A), (short)0, p.M * p.K * 2, 0x00020000);
+    const auto issue = [&](int it, int c, u32x4* r) {
+        const int row0 = it < my_bands ? (gi + it * stride) * BR_BAND + wave * BR_WROWS : p.M;
+        #pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            r[i] = __builtin_amdgcn_raw_buffer_load_b128(rs_a, ((row0 + i * 16 + lane % 16) * p.K + c * 32 + kq * 8) * 2, 0, 0);
+        }
+    };
+    // the first BR_AHEAD chunks (band iteration 0) are in flight while the weights load
+    #pragma unroll
+    for (int u = 0; u < BR_AHEAD; ++u) { issue(0, u, ra[u]); }
+
+    // The weight slice in LDS: 16-byte vector q of row r at slot q ^ (r & 15) of the row, so the 16
+    // consecutive rows one fragment read touches (same q) fall on 16 different slots of a 256-byte
+    // span: distinct banks, with no padding.
+    {  // vector v of the slice is bytes 16 v .. of its rows, row v / (K / 8)
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<uint16_t*>(p.B + (int64_t)n0 * p.K), (short)0, BR_BN * p.K * 2, 0x00020000);
+        const int kv = p.K / 8, total = BR_BN * kv;
+        constexpr int VB = 4096 / THREADS;  // vectors per thread per batch
+        for (int v0 = 0; v0 < total; v0 += THREADS * VB) {
+            u32x4 r[VB];
+            #pragma unroll
+            for (int u = 0; u < VB; ++u) {
+                const int v = v0 + u * THREADS + tid;
+                r[u] = __builtin_amdgcn_raw_buffer_load_b128(rs, v < total ? v * 16 : 0x7ffffff0, 0, 0);
+            }
+            #pragma unroll
+            for (int u = 0; u < VB; ++u) {
+                const int v = v0 + u * THREADS + tid;
+                if (v < total) { Bs[v / kv][(v % kv) ^ ((v / kv) & 15)] = r[u]; }
+            }
+        }
+    }
+    __syncthreads();
+
+    f32x4 acc[4][8];
+    #pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        #pragma unroll
+        for (int j = 0; j < 8; ++j) { acc[i][j] = f32x4{ 0.f, 0.f, 0.f, 0.f }; }
+    }
+    // B fragments of a chunk (the same for every band), from LDS: one chunk ahead when double-buffered
+    constexpr bool B2 = BresShape<WAVES>::b_ahead;
+    bf16x8 fb[B2 ? 2 : 1][8];
+    const auto read_b = [&](int c, bf16x8* f) {
+        #pragma unroll
+        for (int j = 0; j < 8; ++j) { f[j] = __builtin_bit_cast(bf16x8, Bs[j * 16 + lane % 16][(c * 4 + kq) ^ (lane % 16)]); }
+    };
+    if constexpr (B2) { read_b(0, fb[0]); }
+    int lit = 0, lc = BR_AHEAD;  // band iteration and chunk of the next group of loads
+    if (lc == nc) {
+        lc = 0;
+        ++lit;
+    }
+    int it = 0, cc = 0;  // band iteration and chunk being computed
+    while (it < my_bands) {
+        #pragma unroll
+        for (int u = 0; u < BR_AHEAD; ++u) {
+            const int c = cc + u;
+            if constexpr (B2) {
+                read_b(c + 1 == nc ? 0 : c + 1, fb[(u + 1) % 2]);
+            } else {
+                read_b(c, fb[0]);
+            }
+            const int fcur = B2 ? u % 2 : 0;
+            #pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const bf16x8 fa = __builtin_bit_cast(bf16x8, ra[u][i]);
+                #pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[fcur][j], fa, acc[i][j], 0, 0, 0);
+                }
+            }
+            issue(lit, lc + u, ra[u]);
+            // keep the refill here: hoisted below later chunks' MFMAs, the loads would go out in
+            // bursts and leave the ring nearly empty in between
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        lc += BR_AHEAD;
+        if (lc == nc) {
+            lc = 0;
+            ++lit;
+        }
+        cc += BR_AHEAD;
+        if (cc < nc) { continue; }
+
+        // the band is done: lane holds rows n = 4 kq + e of each 16 x 16 result, column m = lane % 16
+        const int row0 = (gi + it * stride) * BR_BAND + wave * BR_WROWS;
+        #pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int m = row0 + i * 16 + lane % 16;
+            if (m >= p.M) { continue; }
+            #pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int n = n0 + j * 16 + 4 * kq;
+                const int64_t at = (int64_t)m * p.N + n;
+                f32x4 v = acc[i][j];
+                if (EPI == EPI_F32) {
+                    *reinterpret_cast<f32x4*>(p.c + at) = v;
+                    continue;
+                }
+                if (EPI == EPI_FWD) {
+                    v += *reinterpret_cast<const f32x4*>(p.bias + n);
+                    if (p.c) { *reinterpret_cast<f32x4*>(p.c + at) = v; }
+                    if (p.relu) {
+                        #pragma unroll
+                        for (int e = 0; e < 4; ++e) { v[e] = fmaxf(v[e], 0.f); }
+                    }
+                } else {
+                    const u32x2 mk = *reinterpret_cast<const u32x2*>(p.mask + at);
+                    #pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        v[e] = bf16_to_f32((uint16_t)(mk[e / 2] >> (16 * (e % 2)))) > 0.f ? v[e] : 0.f;
+                    }
+                }
+                const u32x2 w = { pack_bf16x2(v[0], v[1]), pack_bf16x2(v[2], v[3]) };
+                if (p.out) { *reinterpret_cast<u32x2*>(p.out + at) = w; }
+                if (p.out_t) {
+                    #pragma unroll
+                    for (int e = 0; e < 4; ++e) { p.out_t[(int64_t)(n + e) * p.M + m] = (uint16_t)(w[e / 2] >> (16 * (e % 2))); }
+                }
+            }
+        }
+        #pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            #pragma unroll
+            for (int j = 0; j < 8; ++j) { acc[i][j] = f32x4{ 0.f, 0.f, 0.f, 0.f }; }
+        }
+        cc = 0;
+        ++it;
     }
 }
 
@@ -586,24 +796,6 @@ void launch_nt_tile(int tile, const NtArgs& a, int splits, hipStream_t st) {
     }
 }
 
-// How a K-major product runs: a tile shape (NT_TILES index) and, for a product split over K,
-// where its partials are summed: in the kernel by each tile's last block (choice < NT_NUM_TILES),
-// or by sum_splits_kernel after it (choice - NT_NUM_TILES).  Both sum the partials in split order,
-// so every choice gives the same result.
-struct NtChoice {
-    int tile;
-    bool summed_in_kernel;
-};
-
-NtChoice nt_decode(int choice) { return { choice % NT_NUM_TILES, choice < NT_NUM_TILES }; }
-
-template <int EPI>
-void launch_nt_choice(int choice, NtArgs a, int splits, hipStream_t st) {
-    const NtChoice c = nt_decode(choice);
-    if (!c.summed_in_kernel) { a.counters = nullptr; }
-    launch_nt_tile<EPI>(c.tile, a, splits, st);
-}
-
 // Choice before any timing: the largest tile whose blocks fill the chip twice over (2 per CU), else
 // the smallest; split products summed in the kernel.
 int nt_default_choice(int M, int N, int splits) {
@@ -611,6 +803,76 @@ int nt_default_choice(int M, int N, int splits) {
         if ((int64_t)nt_tiles_of(tile, M, N) * splits >= 512) { return tile; }
     }
     return NT_NUM_TILES - 1;
+}
+
+// How a K-major product runs: a tile shape (NT_TILES index) and, for a product split over K,
+// where its partials are summed: in the kernel by each tile's last block (choice < NT_NUM_TILES),
+// or by sum_splits_kernel after it (choice - NT_NUM_TILES).  Both sum the partials in split order,
+// so every choice gives the same result.  Choice NT_BRES is gemm_nt_bres_kernel (unsplit products
+// of the shapes it takes); it reports tile index NT_NUM_TILES.
+struct NtChoice {
+    int tile;
+    bool summed_in_kernel;
+};
+
+constexpr int NT_BRES = 2 * NT_NUM_TILES;  // 4 waves; NT_BRES + 1: 8 waves
+
+NtChoice nt_decode(int choice) {
+    if (choice >= NT_BRES) { return { NT_NUM_TILES + choice - NT_BRES, true }; }
+    return { choice % NT_NUM_TILES, choice < NT_NUM_TILES };
+}
+
+bool aligned16(const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; }
+
+// Products gemm_nt_bres_kernel takes: unsplit, vector operands and outputs, N a multiple of the
+// 128-column slice, K a multiple of the chunk group (256 with 4 waves, 128 with 8) up to 512, and A
+// within 32-bit byte offsets of one buffer descriptor.
+template <int EPI>
+bool nt_bres_ok(const NtArgs& a, int splits, int waves) {
+    const int group = 32 * (waves == 4 ? BresShape<4>::ahead : BresShape<8>::ahead);
+    return splits == 1 && a.vec_a && a.vec_b && a.vec_out && a.N % BR_BN == 0 && a.K % group == 0 && a.K <= BR_KMAX
+           && (int64_t)a.M * a.K * 2 <= (int64_t)INT32_MAX - (1 << 20) && (EPI != EPI_FWD || aligned16(a.bias));
+}
+
+int device_cus(int device) {
+    static std::atomic<int> cus[64];
+    if (device < 0 || device >= 64) { return 256; }
+    int n = cus[device].load(std::memory_order_relaxed);
+    if (n == 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) { n = 256; }
+        cus[device].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+// Persistent grid of gemm_nt_bres_kernel: XCDS x slices x groups blocks, about one per CU, and no
+// more band groups than bands.
+template <int EPI, int WAVES>
+void launch_nt_bres(int device, const NtArgs& a, hipStream_t st) {
+    const int slices = a.N / BR_BN;
+    const int bands = (a.M + BresShape<WAVES>::band - 1) / BresShape<WAVES>::band;
+    const int groups = std::max(1, std::min(device_cus(device) / (XCDS * slices), (bands + XCDS - 1) / XCDS));
+    hipLaunchKernelGGL((gemm_nt_bres_kernel<EPI, WAVES>), dim3((unsigned)(XCDS * slices * groups)), dim3(64 * WAVES), 0,
+        st, a, groups);
+}
+
+template <int EPI>
+void launch_nt_choice(int device, int choice, NtArgs a, int splits, hipStream_t st) {
+    if (choice >= NT_BRES) {
+        const int waves = choice == NT_BRES ? 4 : 8;
+        if (nt_bres_ok<EPI>(a, splits, waves)) {
+            if (waves == 4) {
+                launch_nt_bres<EPI, 4>(device, a, st);
+            } else {
+                launch_nt_bres<EPI, 8>(device, a, st);
+            }
+            return;
+        }
+        choice = nt_default_choice(a.M, a.N, splits);
+    }
+    const NtChoice c = nt_decode(choice);
+    if (!c.summed_in_kernel) { a.counters = nullptr; }
+    launch_nt_tile<EPI>(c.tile, a, splits, st);
 }
 
 // Autotuner: the first product of each (device, epilogue, outputs, M, N, K, splits) times every
@@ -627,9 +889,13 @@ int nt_choose(int device, const NtArgs& a, int splits, hipStream_t st, bool may_
     const std::array<int64_t, 7> key = { device, EPI, outputs, a.M, a.N, a.K, splits };
     if (!(a.vec_a && a.vec_b)) { return NT_NUM_TILES - 1; }  // element loads: one tile shape
     // PSIM_GEMM_CHOICE forces a choice (tests run every choice and compare the results)
+    const bool bres4 = nt_bres_ok<EPI>(a, splits, 4), bres8 = nt_bres_ok<EPI>(a, splits, 8);
     if (const char* forced = std::getenv("PSIM_GEMM_CHOICE")) {
         const int c = std::atoi(forced);
-        if (c >= 0 && c < (splits > 1 ? 2 * NT_NUM_TILES : NT_NUM_TILES)) { return c; }
+        if ((c >= 0 && c < (splits > 1 ? 2 * NT_NUM_TILES : NT_NUM_TILES)) || (c == NT_BRES && bres4)
+            || (c == NT_BRES + 1 && bres8)) {
+            return c;
+        }
     }
     std::lock_guard<std::mutex> lock(nt_tune_mutex);
     const auto it = nt_tuned.find(key);
@@ -644,27 +910,37 @@ int nt_choose(int device, const NtArgs& a, int splits, hipStream_t st, bool may_
         if (e0) { (void)hipEventDestroy(e0); }
         return fallback;
     }
-    const int choices = splits > 1 ? 2 * NT_NUM_TILES : NT_NUM_TILES;
+    std::vector<int> candidates;
+    for (int c = 0; c < (splits > 1 ? 2 * NT_NUM_TILES : NT_NUM_TILES); ++c) { candidates.push_back(c); }
+    if (bres4) { candidates.push_back(NT_BRES); }
+    if (bres8) { candidates.push_back(NT_BRES + 1); }
     int best = fallback;
     float best_ms = 0.f;
     bool ok = true;
-    for (int choice = 0; choice < choices && ok; ++choice) {
-        launch_nt_choice<EPI>(choice, a, splits, st);
+    for (size_t ci = 0; ci < candidates.size() && ok; ++ci) {
+        const int choice = candidates[ci];
+        launch_nt_choice<EPI>(device, choice, a, splits, st);
         (void)hipEventRecord(e0, st);
-        for (int r = 0; r < 3; ++r) { launch_nt_choice<EPI>(choice, a, splits, st); }
+        for (int r = 0; r < 3; ++r) { launch_nt_choice<EPI>(device, choice, a, splits, st); }
         (void)hipEventRecord(e1, st);
         float ms = 0.f;
         ok = hipGetLastError() == hipSuccess && hipEventSynchronize(e1) == hipSuccess
              && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
-        if (ok && (choice == 0 || ms < best_ms)) {
+        if (ok && (ci == 0 || ms < best_ms)) {
             best = choice;
             best_ms = ms;
         }
         if (ok && std::getenv("PSIM_TUNE_LOG")) {
             const NtChoice c = nt_decode(choice);
-            std::fprintf(stderr, "psim tune: epilogue %d outputs %d %d x %d x %d, %d splits%s, tile %d x %d (%d waves): %.2f us\n",
-                EPI, (int)outputs, a.M, a.N, a.K, splits, splits > 1 ? (c.summed_in_kernel ? " summed in kernel" : " summed after") : "",
-                NT_TILES[c.tile].bm, NT_TILES[c.tile].bn, NT_TILES[c.tile].wgm * NT_TILES[c.tile].wgn, ms * 1e3f / 3.f);
+            if (choice >= NT_BRES) {
+                const int waves = choice == NT_BRES ? 4 : 8;
+                std::fprintf(stderr, "psim tune: epilogue %d outputs %d %d x %d x %d, weights resident, %d x 128 bands (%d waves): %.2f us\n",
+                    EPI, (int)outputs, a.M, a.N, a.K, waves * BR_WROWS, waves, ms * 1e3f / 3.f);
+            } else {
+                std::fprintf(stderr, "psim tune: epilogue %d outputs %d %d x %d x %d, %d splits%s, tile %d x %d (%d waves): %.2f us\n",
+                    EPI, (int)outputs, a.M, a.N, a.K, splits, splits > 1 ? (c.summed_in_kernel ? " summed in kernel" : " summed after") : "",
+                    NT_TILES[c.tile].bm, NT_TILES[c.tile].bn, NT_TILES[c.tile].wgm * NT_TILES[c.tile].wgn, ms * 1e3f / 3.f);
+            }
         }
     }
     (void)hipEventDestroy(e0);
@@ -672,8 +948,6 @@ int nt_choose(int device, const NtArgs& a, int splits, hipStream_t st, bool may_
     if (ok) { nt_tuned[key] = best; }
     return ok ? best : fallback;
 }
-
-bool aligned16(const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; }
 
 // Fills the shape-derived fields of a K-major product and launches it as tuned.  Split products
 // need ws (splits x M x N floats) and counters (NT_COUNTERS, zero); without them, or when the tiles
@@ -698,8 +972,11 @@ int launch_nt(int device, NtArgs a, hipStream_t st, bool may_tune, size_t ws_ele
         }
     }
     a.k_split = k_split;
-    const int choice = nt_choose<EPI>(device, a, splits, st, may_tune);
-    launch_nt_choice<EPI>(choice, a, splits, st);
+    int choice = nt_choose<EPI>(device, a, splits, st, may_tune);
+    if (choice >= NT_BRES && !nt_bres_ok<EPI>(a, splits, choice == NT_BRES ? 4 : 8)) {
+        choice = nt_default_choice(a.M, a.N, splits);
+    }
+    launch_nt_choice<EPI>(device, choice, a, splits, st);
     return nt_decode(choice).tile;
 }
 
@@ -1457,7 +1734,10 @@ int psim_mlp_time_product(psim_mlp* h, int32_t kind, int32_t rows, int32_t reps,
     (void)hipEventDestroy(e1);
     if (!ok) { return mlp_fail(h, PSIM_E_DEVICE, "timing the product failed"); }
     *ms_per_launch = ms / (float)reps;
-    if (tile) { *tile = NT_TILES[used].bm * 1000 + NT_TILES[used].bn; }
+    if (tile) {
+        *tile = used >= NT_NUM_TILES ? -((used == NT_NUM_TILES ? 4 : 8) * BR_WROWS * 1000 + BR_BN)
+                                     : NT_TILES[used].bm * 1000 + NT_TILES[used].bn;
+    }
     return PSIM_OK;
 }
 

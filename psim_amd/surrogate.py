@@ -180,7 +180,10 @@ class Surrogate:
         tile = ctypes.c_int32()
         self._check(self._L.psim_mlp_time_product(self._h, self.PRODUCTS[kind], int(rows), int(reps), ctypes.byref(ms),
                                                   ctypes.byref(tile)))
-        return float(ms.value), f"{tile.value // 1000}x{tile.value % 1000}"
+        code = tile.value
+        if code < 0:  # the kernel that keeps the weight slice in LDS: rows per band x columns
+            return float(ms.value), f"{-code // 1000}x{-code % 1000} (weights resident in LDS)"
+        return float(ms.value), f"{code // 1000}x{code % 1000}"
 
     def train_step(self, x: np.ndarray, t: np.ndarray, lr: float) -> float:
         x = np.ascontiguousarray(x, np.float32)

@@ -55,6 +55,35 @@ def test_gemm_single_tile_identity(surrogate):
     np.testing.assert_array_equal(surrogate.gemm_check(eye, a, False, False), _bf16(a).astype(np.float32))
 
 
+def test_device_bf16_rounding_is_nearest_even(surrogate):
+    """The device's float -> bf16 rounding (one hardware convert, in every kernel that writes bf16)
+    is round-to-nearest-even: exact ties of both parities and signs, values near the largest finite
+    bf16 and tiny normal values.  A network whose output is relu(x0) - relu(-x0) (weights +-1, no
+    bias) returns the device-rounded first feature unchanged."""
+    rng = np.random.default_rng(12)
+    base = rng.uniform(1, 2, size=512).astype(np.float32)
+    u = base.view(np.uint32) & np.uint32(0xFFFF0000)
+    ties = (u | np.uint32(0x8000)).view(np.float32)  # exactly half-way between two bf16 values
+    above = (u | np.uint32(0x8001)).view(np.float32)
+    below = (u | np.uint32(0x7FFF)).view(np.float32)
+    big = rng.uniform(1e37, 3.3e38, size=512).astype(np.float32)
+    tiny = rng.uniform(1.2e-38, 1e-36, size=512).astype(np.float32)
+    vals = np.concatenate([ties, -ties, above, -below, big, -big, tiny, -tiny])
+    net = surrogate.Surrogate(hidden=64, layers=1, seed=2)
+    p = np.zeros(net.n_params, np.float32)
+    (w1, b1), (w2, b2) = net.layer_views(p)
+    w1[0, 0], w1[0, 1] = 1.0, -1.0
+    w2[0, 0], w2[1, 0] = 1.0, -1.0
+    net.set_params(p)
+    x = np.zeros((vals.size, surrogate.N_FEATURES), np.float32)
+    x[:, 0] = vals
+    got = net.predict(x)
+    net.close()
+    w = vals.view(np.uint32).astype(np.uint64)
+    want = (((w + 0x7FFF + ((w >> 16) & 1)) >> 16) << 16).astype(np.uint32)
+    np.testing.assert_array_equal(got.view(np.uint32), want)
+
+
 def _reference_gradients(net, params, x, t):
     """Loss and flat gradient in float64 on the CPU, with the GPU's bf16 roundings in the forward."""
     views = net.layer_views(params.astype(np.float32))
@@ -179,7 +208,7 @@ def test_every_tile_and_split_sum_gives_the_same_bits(surrogate, monkeypatch):
     t = rng.uniform(-1, 1, size=2048).astype(np.float32)
     net = surrogate.Surrogate(seed=4)
     results = []
-    for choice in range(10):
+    for choice in range(12):  # 5 tiles, 5 tiles with the separate split sum, resident weights (4, 8 waves)
         monkeypatch.setenv("PSIM_GEMM_CHOICE", str(choice))
         fwd = surrogate.gemm_check(a, b, False, True)
         split = surrogate.gemm_check(aw, bw, False, True)  # 512 x 512 x 4096: split over K
@@ -192,6 +221,50 @@ def test_every_tile_and_split_sum_gives_the_same_bits(surrogate, monkeypatch):
         np.testing.assert_array_equal(split, results[0][1])
         np.testing.assert_array_equal(y, results[0][2])
         np.testing.assert_array_equal(grad, results[0][3])
+
+
+@pytest.mark.parametrize("shape", [(1, 128, 256), (257, 640, 256), (1000, 512, 512), (4100, 384, 512)],
+                         ids=["one-row", "ragged-band", "rows-1000", "three-slices"])
+@pytest.mark.parametrize("choice", ["10", "11"], ids=["4-waves", "8-waves"])
+def test_resident_weight_kernel_matches_the_tiles(surrogate, monkeypatch, shape, choice):
+    """The kernel that keeps the weight slice in LDS (PSIM_GEMM_CHOICE 10 with 4 waves, 11 with 8; it
+    takes N a multiple of 128 and K of 256 up to 512, as every shape here) gives the 64 x 64 tile's
+    bits on K-major products whose rows end inside a band, matches the host product, and puts every
+    element in its place (identity operand)."""
+    rng = np.random.default_rng(31)
+    M, N, K = shape
+    a = rng.normal(size=(M, K)).astype(np.float32)
+    b = rng.normal(size=(N, K)).astype(np.float32)
+    monkeypatch.setenv("PSIM_GEMM_CHOICE", "4")
+    tiled = surrogate.gemm_check(a, b, False, True)
+    monkeypatch.setenv("PSIM_GEMM_CHOICE", choice)
+    resident = surrogate.gemm_check(a, b, False, True)
+    np.testing.assert_array_equal(resident, tiled)
+    want = _bf16(a) @ _bf16(b).T
+    assert np.max(np.abs(resident - want)) < 1e-4 * np.sqrt(K) * np.max(np.abs(want))
+    eye = np.eye(N, K, dtype=np.float32)  # B = [I; 0]: C = A's first N columns
+    got = surrogate.gemm_check(a, eye, False, True)
+    np.testing.assert_array_equal(got[:, :min(N, K)], _bf16(a)[:, :min(N, K)].astype(np.float32))
+    if N > K:
+        assert not got[:, K:].any()
+
+
+def test_resident_weight_kernel_in_the_network(surrogate, monkeypatch):
+    """The network's forward and gradients at 300 rows (the last band ragged) are the same bits with
+    the resident-weight kernel, at 4 and 8 waves, as with the 64 x 64 tile."""
+    rng = np.random.default_rng(33)
+    x = rng.uniform(-1, 1, size=(300, surrogate.N_FEATURES)).astype(np.float32)
+    t = rng.uniform(-1, 1, size=300).astype(np.float32)
+    net = surrogate.Surrogate(seed=5)
+    out = []
+    for choice in ("4", "10", "11"):
+        monkeypatch.setenv("PSIM_GEMM_CHOICE", choice)
+        out.append((net.predict(x), net.gradients(x, t)))
+    net.close()
+    for o in out[1:]:
+        np.testing.assert_array_equal(out[0][0], o[0])
+        assert out[0][1][0] == o[1][0]
+        np.testing.assert_array_equal(out[0][1][1], o[1][1])
 
 
 def test_graph_replayed_training_steps_equal_eager_steps(surrogate):
