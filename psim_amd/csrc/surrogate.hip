@@ -564,6 +564,23 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC) gemm_nt_kernel(NtArgs p) 
     }
 }
 
+// 4 x 4 transpose of 16-bit values within each quad of lanes: lane 4a + b passes row b of its
+// quad's block (elements 0..3, two per word, low half first) and gets column b back.  Two DPP quad
+// permutations: lanes b and b ^ 2 swap the 2 x 2 blocks each needs, then lanes b and b ^ 1 swap
+// halves within them.  Every lane of the wave must be active.
+__device__ inline u32x2 quad_transpose16(u32x2 w, int lane) {
+    const int b = lane & 3;
+    const uint32_t give = b < 2 ? w[1] : w[0];
+    const uint32_t got = (uint32_t)__builtin_amdgcn_mov_dpp((int)give, 0x4E, 0xF, 0xF, false);  // quad_perm 2, 3, 0, 1
+    const uint32_t A = b < 2 ? w[0] : got;  // rows b & 1 and (b & 1) + 2 of columns 2 (b / 2), + 1
+    const uint32_t B = b < 2 ? got : w[1];
+    const bool even = (b & 1) == 0;
+    const uint32_t send = even ? ((A >> 16) | (B & 0xffff0000u)) : ((A & 0xffffu) | (B << 16));
+    const uint32_t r = (uint32_t)__builtin_amdgcn_mov_dpp((int)send, 0xB1, 0xF, 0xF, false);  // quad_perm 1, 0, 3, 2
+    return even ? u32x2{ (A & 0xffffu) | (r << 16), (B & 0xffffu) | (r & 0xffff0000u) }
+                : u32x2{ (r & 0xffffu) | (A & 0xffff0000u), (r >> 16) | (B & 0xffff0000u) };
+}
+
 // K-major products with the weight slice resident in LDS (N a multiple of 128, K a multiple of 128
 // or 256 up to 512: the network's forward and input-gradient products).  The tiles above stage
 // both operands through LDS at every K step, and at 64 x 32 per wave they read about one 16-byte
@@ -575,7 +592,8 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC) gemm_nt_kernel(NtArgs p) 
 // through the bands dealt to it, the slices of one band on the same XCD, so A is fetched from HBM
 // once and shared in L2.  Each output element sums the same 32-deep chunks, in the same order and
 // the same lane layout, as in gemm_nt_kernel; the MFMA takes B as its first operand, so a lane
-// holds 4 consecutive columns of one row (16-byte FP32, 8-byte bf16 row stores).
+// holds 4 consecutive columns of one row (16-byte FP32, 8-byte bf16 row stores; the transposed
+// copy goes through a lane-quad transpose into 8-byte pieces of its rows).
 constexpr int BR_BN = 128;  // columns of the resident weight slice
 constexpr int BR_WROWS = 64;  // rows per wave and band
 constexpr int BR_KMAX = 512;
@@ -701,40 +719,53 @@ __global__ void __launch_bounds__(64 * WAVES, WAVES / 4) gemm_nt_bres_kernel(NtA
         cc += BR_AHEAD;
         if (cc < nc) { continue; }
 
-        // the band is done: lane holds rows n = 4 kq + e of each 16 x 16 result, column m = lane % 16
+        // The band is done: lane holds rows n = 4 kq + e of each 16 x 16 result, column m = lane % 16.
+        // Every lane runs the epilogue (the transposed copy's lane exchange needs all of them);
+        // rows past M only skip their loads and stores.
         const int row0 = (gi + it * stride) * BR_BAND + wave * BR_WROWS;
         #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int m = row0 + i * 16 + lane % 16;
-            if (m >= p.M) { continue; }
+            const bool in_rows = m < p.M;
             #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int n = n0 + j * 16 + 4 * kq;
                 const int64_t at = (int64_t)m * p.N + n;
                 f32x4 v = acc[i][j];
                 if (EPI == EPI_F32) {
-                    *reinterpret_cast<f32x4*>(p.c + at) = v;
+                    if (in_rows) { *reinterpret_cast<f32x4*>(p.c + at) = v; }
                     continue;
                 }
                 if (EPI == EPI_FWD) {
                     v += *reinterpret_cast<const f32x4*>(p.bias + n);
-                    if (p.c) { *reinterpret_cast<f32x4*>(p.c + at) = v; }
+                    if (p.c && in_rows) { *reinterpret_cast<f32x4*>(p.c + at) = v; }
                     if (p.relu) {
                         #pragma unroll
                         for (int e = 0; e < 4; ++e) { v[e] = fmaxf(v[e], 0.f); }
                     }
                 } else {
-                    const u32x2 mk = *reinterpret_cast<const u32x2*>(p.mask + at);
+                    const u32x2 mk = in_rows ? *reinterpret_cast<const u32x2*>(p.mask + at) : u32x2{ 0u, 0u };
                     #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         v[e] = bf16_to_f32((uint16_t)(mk[e / 2] >> (16 * (e % 2)))) > 0.f ? v[e] : 0.f;
                     }
                 }
                 const u32x2 w = { pack_bf16x2(v[0], v[1]), pack_bf16x2(v[2], v[3]) };
-                if (p.out) { *reinterpret_cast<u32x2*>(p.out + at) = w; }
+                if (p.out && in_rows) { *reinterpret_cast<u32x2*>(p.out + at) = w; }
                 if (p.out_t) {
-                    #pragma unroll
-                    for (int e = 0; e < 4; ++e) { p.out_t[(int64_t)(n + e) * p.M + m] = (uint16_t)(w[e / 2] >> (16 * (e % 2))); }
+                    // after the quad transpose, lane 4a + b holds column n + b (of the quad's 4 x 4
+                    // block) for rows m4 .. m4 + 3: one 8-byte piece of a transposed row
+                    const u32x2 t = quad_transpose16(w, lane);
+                    const int m4 = row0 + i * 16 + (lane % 16 & ~3);
+                    uint16_t* dst = p.out_t + (int64_t)(n + (lane & 3)) * p.M + m4;
+                    if (p.vec_out_t && m4 + 4 <= p.M) {
+                        *reinterpret_cast<u32x2*>(dst) = t;
+                    } else {
+                        #pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            if (m4 + e < p.M) { dst[e] = (uint16_t)(t[e / 2] >> (16 * (e % 2))); }
+                        }
+                    }
                 }
             }
         }
