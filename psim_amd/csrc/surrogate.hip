@@ -613,7 +613,8 @@ template <int EPI, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES, WAVES / 4) gemm_nt_bres_kernel(NtArgs p, int groups) {
     constexpr int BR_BAND = BresShape<WAVES>::band, BR_AHEAD = BresShape<WAVES>::ahead, THREADS = 64 * WAVES;
     __shared__ u32x4 Bs[BR_BN][BR_PITCH];
-    // the wave index is uniform: read it as a scalar, so the A descriptors built from it are too
+    // the wave index is uniform: read it as a scalar, so the row bases of the A loads built from it
+    // stay in scalar registers (a VGPR-held base wrapped every load in a readfirstlane loop)
     const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid / 64), lane = tid % 64, kq = lane / 16;
     // block -> (column slice, band group): the host launches XCDS x slices x groups blocks; blocks
     // are dealt to the XCDs in turn, and the slices of a band group sit on one XCD
