@@ -77,8 +77,8 @@ int psim_mlp_time_gemm(int device, int32_t ta, int32_t tb, int32_t M, int32_t N,
  *   kind 2  weight gradient: transposed activations x transposed gradient, split over the rows;
  *   kind 3  input gradient through the ReLU: bf16 gradient out, row-major and transposed.
  * *tile (optional) receives the tile the autotuner chose, as 1000 x rows + columns (128128 ...),
- * negated for the kernel that keeps its 128-column weight slice in LDS over 256-row bands
- * (-256128).
+ * negated for the kernel that keeps its 128-column weight slice in LDS, with its band's rows:
+ * -256128 (4 waves, 256-row bands) or -512128 (8 waves, 512-row bands).
  * Needs at least two hidden layers. */
 int psim_mlp_time_product(psim_mlp* mlp, int32_t kind, int32_t rows, int32_t reps, float* ms_per_launch,
     int32_t* tile);
