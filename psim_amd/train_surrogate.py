@@ -238,7 +238,8 @@ def cmd_evaluate(args) -> None:
     import psim_amd
     from . import surrogate
 
-    params = structures(EVAL_SEED, args.count, args.max_wall_diff)
+    # held-out structures [first, first + count) of the family (an evaluation can run in parts)
+    params = structures(EVAL_SEED, args.first + args.count, args.max_wall_diff)[args.first:]
     engine = psim_amd.Engine(args.device)
     net = surrogate.load_weights(args.weights or surrogate.WEIGHTS_PATH, args.device)
     headline = ABS_KEY if args.max_wall_diff is not None else str(HEADLINE_TOL)
@@ -283,7 +284,8 @@ def main(argv=None) -> int:
     ap.add_argument("--count", type=int, default=None)
     ap.add_argument("--phonons", type=int, default=None)
     ap.add_argument("--chunk", type=int, default=1024)
-    ap.add_argument("--first", type=int, default=0)
+    ap.add_argument("--first", type=int, default=0,
+                    help="data: first training structure; evaluate: first held-out structure")
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=1e-3)
