@@ -24,13 +24,14 @@ def surrogate():
     return s
 
 
-@pytest.mark.parametrize("shape", [(70, 45, 100), (72, 40, 104), (4100, 4104, 64), (70, 45, 1100)],
-                         ids=["ragged", "vector-loads", "large-tiles", "split-k"])
+@pytest.mark.parametrize("shape", [(70, 45, 100), (72, 40, 104), (4100, 4104, 64), (70, 45, 1100), (512, 512, 8192)],
+                         ids=["ragged", "vector-loads", "large-tiles", "split-k", "split-k-16"])
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
 def test_gemm_matches_host_product(surrogate, ta, tb, shape):
     """Ragged shapes load element by element, shapes whose rows are multiples of 8 load 16-byte
     vectors, products of at least 1024 128 x 128 blocks use the large tile, and a product of few
-    blocks with K >= 1024 is split over K and summed in order."""
+    blocks with K >= 1024 is split over K and summed in order (512 x 512 x 8192, K-major: 16
+    splits)."""
     rng = np.random.default_rng(3 + 2 * ta + tb)
     M, N, K = shape
     a = rng.normal(size=(K, M) if ta else (M, K)).astype(np.float32)
@@ -203,6 +204,8 @@ def test_every_tile_and_split_sum_gives_the_same_bits(surrogate, monkeypatch):
     b = rng.normal(size=(512, 512)).astype(np.float32)
     aw = rng.normal(size=(512, 4096)).astype(np.float32)
     bw = rng.normal(size=(512, 4096)).astype(np.float32)
+    aw16 = rng.normal(size=(512, 8192)).astype(np.float32)
+    bw16 = rng.normal(size=(512, 8192)).astype(np.float32)
     # 2048 rows: the hidden layers' weight gradients (K = rows) are split over K
     x = rng.uniform(-1, 1, size=(2048, surrogate.N_FEATURES)).astype(np.float32)
     t = rng.uniform(-1, 1, size=2048).astype(np.float32)
@@ -211,16 +214,15 @@ def test_every_tile_and_split_sum_gives_the_same_bits(surrogate, monkeypatch):
     for choice in range(12):  # 5 tiles, 5 tiles with the separate split sum, resident weights (4, 8 waves)
         monkeypatch.setenv("PSIM_GEMM_CHOICE", str(choice))
         fwd = surrogate.gemm_check(a, b, False, True)
-        split = surrogate.gemm_check(aw, bw, False, True)  # 512 x 512 x 4096: split over K
+        split = surrogate.gemm_check(aw, bw, False, True)  # 512 x 512 x 4096: 8 splits over K
+        split16 = surrogate.gemm_check(aw16, bw16, False, True)  # 512 x 512 x 8192: 16 splits
         y = net.predict(x)
         loss, grad = net.gradients(x, t)
-        results.append((fwd, split, y, grad))
+        results.append((fwd, split, split16, y, grad))
     net.close()
-    for fwd, split, y, grad in results[1:]:
-        np.testing.assert_array_equal(fwd, results[0][0])
-        np.testing.assert_array_equal(split, results[0][1])
-        np.testing.assert_array_equal(y, results[0][2])
-        np.testing.assert_array_equal(grad, results[0][3])
+    for r in results[1:]:
+        for got, want in zip(r, results[0]):
+            np.testing.assert_array_equal(got, want)
 
 
 @pytest.mark.parametrize("shape", [(1, 128, 256), (257, 640, 256), (1000, 512, 512), (4100, 384, 512)],
