@@ -112,11 +112,11 @@ def _reference_gradients(net, params, x, t):
     return float(loss.detach()), np.concatenate(flat)
 
 
-@pytest.mark.parametrize("rows", [150, 2100])
+@pytest.mark.parametrize("rows", [150, 2100, 8192])
 @pytest.mark.parametrize("hidden,layers", [(96, 3), (512, 4)])
 def test_gradients_match_cpu_reference(surrogate, rows, hidden, layers):
-    """At 2100 rows the weight gradients (K = rows) are split over K.  512 x 4 is the warm-start
-    network's own shape (32 -> 512 x 4 -> 1)."""
+    """At 2100 rows the weight gradients (K = rows) are split over K 4 ways, at 8192 rows 16 ways.
+    512 x 4 is the warm-start network's own shape (32 -> 512 x 4 -> 1)."""
     net = surrogate.Surrogate(n_in=32, hidden=hidden, layers=layers, seed=1)
     rng = np.random.default_rng(5)
     x = rng.normal(size=(rows, 32)).astype(np.float32)
