@@ -39,8 +39,9 @@ def test_warm_start_at_the_absolute_criterion(setup):
     below 0.05 K for 3 windows.  Held-out structures whose walls differ by at most 20 K (where 3.2e7
     phonons keep the window-mean noise below the tolerance), each from a cold start, the
     linear-in-arc-length guess and the network's prediction.  On 64 such structures the warm start
-    reaches steady state 3.06x sooner in simulated time and 2.73x sooner in GPU time (3.45x and
-    2.81x on 64 with walls within 30 K at 6.4e7 phonons; profiles/surrogate_warm_start_abs.md);
+    reaches steady state 3.06x sooner in simulated time and 2.73x sooner in GPU time, on the first
+    512 of them 3.49x and 2.57x (3.45x and 2.81x on 64 with walls within 30 K at 6.4e7 phonons;
+    profiles/surrogate_warm_start_abs.md);
     this test runs 16 of the 20 K ones (2.29x and 1.69x with seed 13:
     the window-mean noise is a few hundredths of a kelvin, so one structure that lingers near the
     tolerance moves a 16-structure total).  At this tolerance the linear guess, 0.2-0.3 K RMS from the
