@@ -373,10 +373,10 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16
 # HBM bytes of one 65536 x 512 x 512 fused forward product, per kernel the autotuner may pick for it,
 # copied from the counter profiles (FETCH_SIZE x 1.9910 + WRITE_SIZE x 0.9666, the copy-kernel
 # calibration, per dispatch; counters cannot be read in this process):
-# - weights resident in LDS, 8 waves: 36017 KiB and 67584 KiB (profiles/gemm_nt_bres_kernel_surrogate.md);
+# - weights resident in LDS, 8 waves: 34992 KiB and 65536 KiB (profiles/gemm_nt_bres_kernel_surrogate.md);
 # - the 128 x 128 tile: 39469 KiB and 83280 KiB (profiles/gemm_nt_kernel_surrogate.md).
 GEMM_FWD_65536_TRAFFIC = {
-    "512x128 (weights resident in LDS)": (140.3e6, "gemm_nt_bres_kernel", "profiles/gemm_nt_bres_kernel_surrogate.md"),
+    "512x128 (weights resident in LDS)": (136.2e6, "gemm_nt_bres_kernel", "profiles/gemm_nt_bres_kernel_surrogate.md"),
     "128x128": (162.9e6, "gemm_nt_kernel", "profiles/gemm_nt_kernel_surrogate.md"),
 }
 

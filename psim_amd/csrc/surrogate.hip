@@ -581,7 +581,7 @@ __device__ inline u32x2 quad_transpose16(u32x2 w, int lane) {
                 : u32x2{ (r & 0xffffu) | (A & 0xffff0000u), (r >> 16) | (B & 0xffff0000u) };
 }
 
-// K-major products with the weight slice resident in LDS (N a multiple of 128, K a multiple of 128
+// K-major products with the weight slice resident in LDS (N a multiple of 128, K a multiple of 64
 // or 256 up to 512: the network's forward and input-gradient products).  The tiles above stage
 // both operands through LDS at every K step, and at 64 x 32 per wave they read about one 16-byte
 // LDS vector per MFMA.  Here a block loads rows n0 .. n0 + 127 of B (all of K, 128 KB) into LDS
@@ -600,12 +600,13 @@ constexpr int BR_KMAX = 512;
 constexpr int BR_PITCH = BR_KMAX / 8;  // LDS row pitch in 16-byte vectors (1024 bytes), slots swizzled
 
 // WAVES 4: one wave per SIMD (512 registers), eight chunks of A ahead, B fragments read a chunk
-// ahead.  WAVES 8: two waves per SIMD (256 registers each), four chunks ahead, each wave's waits
-// covered by the other's MFMAs.  A band is WAVES x 64 rows.
+// ahead.  WAVES 8: two waves per SIMD (256 registers each), two chunks ahead, each wave's waits
+// covered by the other's MFMAs (a four-chunk ring spilled 20 bytes per lane in the forward and
+// took 58.2 us against 56.6 us on 65536 x 512 x 512).  A band is WAVES x 64 rows.
 template <int WAVES>
 struct BresShape {
     static constexpr int band = WAVES * BR_WROWS;
-    static constexpr int ahead = WAVES == 4 ? 8 : 4;  // 32-deep chunks of A in flight per wave
+    static constexpr int ahead = WAVES == 4 ? 8 : 2;  // 32-deep chunks of A in flight per wave
     static constexpr bool b_ahead = WAVES == 4;  // B fragments double-buffered
 };
 
@@ -861,7 +862,7 @@ NtChoice nt_decode(int choice) {
 bool aligned16(const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; }
 
 // Products gemm_nt_bres_kernel takes: unsplit, vector operands and outputs, N a multiple of the
-// 128-column slice, K a multiple of the chunk group (256 with 4 waves, 128 with 8) up to 512, and A
+// 128-column slice, K a multiple of the chunk group (256 with 4 waves, 64 with 8) up to 512, and A
 // within 32-bit byte offsets of one buffer descriptor.
 template <int EPI>
 bool nt_bres_ok(const NtArgs& a, int splits, int waves) {

@@ -225,14 +225,15 @@ def test_every_tile_and_split_sum_gives_the_same_bits(surrogate, monkeypatch):
             np.testing.assert_array_equal(got, want)
 
 
-@pytest.mark.parametrize("shape", [(1, 128, 256), (257, 640, 256), (1000, 512, 512), (4100, 384, 512)],
-                         ids=["one-row", "ragged-band", "rows-1000", "three-slices"])
+@pytest.mark.parametrize("shape", [(1, 128, 256), (257, 640, 256), (1000, 512, 512), (4100, 384, 512), (300, 256, 192)],
+                         ids=["one-row", "ragged-band", "rows-1000", "three-slices", "k-192"])
 @pytest.mark.parametrize("choice", ["10", "11"], ids=["4-waves", "8-waves"])
 def test_resident_weight_kernel_matches_the_tiles(surrogate, monkeypatch, shape, choice):
     """The kernel that keeps the weight slice in LDS (PSIM_GEMM_CHOICE 10 with 4 waves, 11 with 8; it
-    takes N a multiple of 128 and K of 256 up to 512, as every shape here) gives the 64 x 64 tile's
-    bits on K-major products whose rows end inside a band, matches the host product, and puts every
-    element in its place (identity operand)."""
+    takes N a multiple of 128 and K a multiple of 256 with 4 waves, of 64 with 8, up to 512: K = 192
+    runs on the 8-wave kernel alone, the 4-wave choice falling back to a tile) gives the 64 x 64
+    tile's bits on K-major products whose rows end inside a band, matches the host product, and puts
+    every element in its place (identity operand)."""
     rng = np.random.default_rng(31)
     M, N, K = shape
     a = rng.normal(size=(M, K)).astype(np.float32)
