@@ -268,6 +268,7 @@ void Material::phononDist(double temp, bool ta, double* out) const {  // :184-20
 }
 
 void Material::initializeTables(double low, double high) {  // :101-123
+    if (built_ && low == built_low_ && high == built_high_ && full_ == built_full_) { return; }
     const auto num_steps = static_cast<std::size_t>((high - low) / static_cast<double>(TEMP_INTERVAL));
     temps_.assign(num_steps, 0.);
     double n = -1.;
@@ -281,6 +282,10 @@ void Material::initializeTables(double low, double high) {  // :101-123
         base_sums_[i] = std::accumulate(la.begin(), la.end(), 0.) + std::accumulate(ta.begin(), ta.end(), 0.);
     }
     cache_.clear();
+    built_ = true;
+    built_full_ = full_;
+    built_low_ = low;
+    built_high_ = high;
 }
 
 std::size_t Material::tempIndex(double temp) const noexcept {  // :241-246

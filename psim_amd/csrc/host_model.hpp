@@ -93,6 +93,10 @@ private:
     std::vector<double> temps_;
     std::vector<double> base_sums_;
     mutable std::map<std::size_t, std::shared_ptr<Table>> cache_;
+    // bounds and mode the tables were last built for: the tables are a function of these and of
+    // the material's constants alone, so a run with the same ones keeps them (and the cache)
+    bool built_ = false, built_full_ = false;
+    double built_low_ = 0., built_high_ = 0.;
     void phononDist(double temp, bool ta, double* out) const;
 };
 

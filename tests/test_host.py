@@ -53,6 +53,8 @@ def test_engine_reports_missing_gpu_cleanly():
     configs.linear_bar(num_phonons=30_000, spec=0.4, sim_type=1, step_interval=20, t_eq=0.0).to_dict(),
 ], ids=["linear_ss", "lshape", "film", "bar_periodic_full"])
 def test_host_budget_equals_oracle(cfg):
+    """One host model prepared for several seeds in turn (its material tables built once and kept,
+    the temperature bounds being the same) gives each seed's budget as a fresh oracle model does."""
     host = psim_amd.Model(cfg)
     for seed in [0, 1, 12345, 2**40 + 7]:
         eff, total, counts = host.prepare(seed)
